@@ -1,0 +1,526 @@
+// abi.cpp -- C-ABI of libhclpp_hip.so (declared in include/hclpp_hip.h).
+//
+// Owns the context: grid tables in device memory (node coordinates, cell
+// sizes and centres, trigonometric tables computed once on the host), the
+// equation of state, the stream, reduction scratch and the device clock of
+// the fused step.  No exception crosses the C boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fused.hpp"
+#include "hclpp_hip.h"
+#include "launch.hpp"
+
+namespace hclpp {
+// reference-shaped kernels reading conservative arrays (ref_kernels.hip)
+void launch_time_step_cons(int nd, int geom, const GridDev& G, const Range3& r, const PerfectGas& eos, const double* rho, const double* rhou,
+                           const double* E, double* partials, double* out, hipStream_t s);
+}  // namespace hclpp
+
+using namespace hclpp;
+
+struct hclpp_ctx {
+    hclpp_config cfg;
+    GridDev G;
+    PerfectGas eos;
+    BcSpec bc;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    double* tables = nullptr;       // all grid tables, one allocation
+    double* partials = nullptr;     // reduction partials
+    int64_t partials_cap = 0;       // doubles
+    double* ghost_partials = nullptr;
+    int64_t ghost_cap = 0;
+    double* scalars = nullptr;      // device scalars (reduction outputs)
+    double* host_scalars = nullptr; // pinned
+    DevClock* clock = nullptr;
+    DevClock* host_clock = nullptr; // pinned
+    double* fx_old = nullptr;       // Hancock scratch for passive scalars
+    int64_t fx_old_cap = 0;
+    GhostShell shell{};
+    int64_t fused_nb = 0;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(HCLPP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HCLPP_HIP(call)                                  \
+    do {                                                 \
+        hipError_t const e_ = (call);                    \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+#define HCLPP_LAUNCHED(what)                                \
+    do {                                                    \
+        hipError_t const e_ = hipGetLastError();            \
+        if (e_ != hipSuccess) return hip_fail(e_, what);    \
+    } while (0)
+
+Range3 to_range(const hclpp_range* r)
+{
+    Range3 o;
+    for (int d = 0; d < 3; ++d) {
+        o.b[d] = r->begin[d];
+        o.e[d] = r->end[d];
+    }
+    return o;
+}
+
+int check_range(const hclpp_ctx* c, const hclpp_range* r, int halo)
+{
+    if (!r) return fail(HCLPP_ERR_ARG, "null range");
+    for (int d = 0; d < 3; ++d) {
+        int const h = d < c->cfg.ndim ? halo : 0;
+        if (r->begin[d] - h < 0 || r->end[d] + h > c->G.nwg[d] || r->begin[d] > r->end[d])
+            return fail(HCLPP_ERR_ARG, "range outside the grid (with the operator's stencil)");
+    }
+    return HCLPP_OK;
+}
+
+int ensure_partials(hclpp_ctx* c, int64_t n)
+{
+    if (n <= c->partials_cap) return HCLPP_OK;
+    if (c->partials) HCLPP_HIP(hipFree(c->partials));
+    c->partials = nullptr;
+    HCLPP_HIP(hipMalloc(&c->partials, sizeof(double) * n));
+    c->partials_cap = n;
+    return HCLPP_OK;
+}
+
+int read_scalar(hclpp_ctx* c, double* out)
+{
+    HCLPP_HIP(hipMemcpyAsync(c->host_scalars, c->scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HCLPP_HIP(hipStreamSynchronize(c->stream));
+    *out = c->host_scalars[0];
+    return HCLPP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hclpp_last_error(void) { return g_error.c_str(); }
+
+int hclpp_limiter_from_name(const char* name)
+{
+    if (!name) return -1;
+    if (!std::strcmp(name, "Constant")) return HCLPP_LIM_CONSTANT;
+    if (!std::strcmp(name, "VanLeer")) return HCLPP_LIM_VANLEER;
+    if (!std::strcmp(name, "Minmod")) return HCLPP_LIM_MINMOD;
+    if (!std::strcmp(name, "VanAlbada")) return HCLPP_LIM_VANALBADA;
+    return -1;
+}
+
+int hclpp_riemann_from_name(const char* name)
+{
+    if (!name) return -1;
+    if (!std::strcmp(name, "HLL")) return HCLPP_RS_HLL;
+    if (!std::strcmp(name, "HLLC")) return HCLPP_RS_HLLC;
+    if (!std::strcmp(name, "Low Mach")) return HCLPP_RS_LOWMACH;
+    return -1;
+}
+
+int hclpp_bc_from_name(const char* name)
+{
+    if (!name) return -1;
+    if (!std::strcmp(name, "Periodic")) return HCLPP_BC_PERIODIC;
+    if (!std::strcmp(name, "NullGradient")) return HCLPP_BC_NULLGRADIENT;
+    if (!std::strcmp(name, "Reflexive")) return HCLPP_BC_REFLEXIVE;
+    return -1;
+}
+
+int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out)
+{
+    if (!cfg || !out) return fail(HCLPP_ERR_ARG, "null argument");
+    *out = nullptr;
+    int const nd = cfg->ndim;
+    if (nd < 1 || nd > 3) return fail(HCLPP_ERR_ARG, "ndim must be 1, 2 or 3");
+    // thermodynamics/perfect_gas.cpp:15-21
+    if (!std::isfinite(cfg->gamma) || cfg->gamma <= 1) return fail(HCLPP_ERR_ARG, "Invalid gamma");
+    if (!std::isfinite(cfg->mmw) || cfg->mmw <= 0) return fail(HCLPP_ERR_ARG, "Invalid mmw");
+    if (cfg->geom != HCLPP_GEOM_CARTESIAN && cfg->geom != HCLPP_GEOM_SPHERICAL) return fail(HCLPP_ERR_ARG, "unknown geometry");
+    if (cfg->limiter < 0 || cfg->limiter > 3) return fail(HCLPP_ERR_ARG, "unknown limiter");
+    if (cfg->riemann < 0 || cfg->riemann > 2) return fail(HCLPP_ERR_ARG, "unknown Riemann solver");
+    if (cfg->gravity != HCLPP_GRAV_UNIFORM && cfg->gravity != HCLPP_GRAV_POINT_MASS) return fail(HCLPP_ERR_ARG, "unknown gravity");
+    if (cfg->nfx < 0) return fail(HCLPP_ERR_ARG, "nfx < 0");
+    if (cfg->ng < 2) return fail(HCLPP_ERR_ARG, "ng must be >= 2 (the reconstruction stencil is two cells wide)");
+    if (cfg->precision != HCLPP_FP64 && cfg->precision != HCLPP_FP32_STORAGE) return fail(HCLPP_ERR_ARG, "unknown precision");
+    int seen[6] = {0, 0, 0, 0, 0, 0};
+    for (int f = 0; f < 2 * nd; ++f) {
+        if (cfg->bc[f] < 0 || cfg->bc[f] > 2) return fail(HCLPP_ERR_ARG, "unknown boundary condition");
+        int const o = cfg->bc_order[f];
+        if (o < 0 || o >= 2 * nd || seen[o]) return fail(HCLPP_ERR_ARG, "boundary priority not fully defined");
+        seen[o] = 1;
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (!cfg->x[d]) return fail(HCLPP_ERR_ARG, "null node coordinates");
+        if (d < nd && cfg->n[d] < cfg->ng) return fail(HCLPP_ERR_ARG, "interior extent smaller than the ghost width");
+        if (d >= nd && cfg->n[d] != 1) return fail(HCLPP_ERR_ARG, "inactive dims must have extent 1");
+    }
+
+    auto* c = new hclpp_ctx();
+    c->cfg = *cfg;
+    for (int d = 0; d < 3; ++d) c->cfg.x[d] = nullptr;
+    GridDev& G = c->G;
+    for (int d = 0; d < 3; ++d) {
+        G.n[d] = cfg->n[d];
+        G.ngh[d] = d < nd ? cfg->ng : 0;  // mesh/grid.cpp:56-58
+        G.nwg[d] = G.n[d] + 2 * G.ngh[d];
+    }
+    G.plane = (int64_t)G.nwg[0] * G.nwg[1] * G.nwg[2];
+    for (int d = 0; d < 3; ++d) G.g[d] = cfg->g[d];
+    G.gravity = cfg->gravity;
+    c->eos.gamma = cfg->gamma;
+    c->eos.gamma_m1 = cfg->gamma - 1;  // perfect_gas.cpp:13
+    c->eos.mmw = cfg->mmw;
+    for (int f = 0; f < 6; ++f) c->bc.kind[f] = f < 2 * nd ? cfg->bc[f] : 0;
+
+    // host tables (mesh/grid.cpp:31-44, :134-155; geometry.cpp trigonometry; gravity.cpp:44-55)
+    std::vector<double> h;
+    size_t off_x[3], off_dx[3], off_xc[3];
+    for (int d = 0; d < 3; ++d) {
+        int const nn = G.nwg[d];
+        off_x[d] = h.size();
+        h.insert(h.end(), cfg->x[d], cfg->x[d] + nn + 1);
+        off_dx[d] = h.size();
+        for (int i = 0; i < nn; ++i) h.push_back(cfg->x[d][i + 1] - cfg->x[d][i]);
+        off_xc[d] = h.size();
+        for (int i = 0; i < nn; ++i) h.push_back((cfg->x[d][i] + cfg->x[d][i + 1]) / 2);
+    }
+    int const n1 = G.nwg[1];
+    double const* x1 = cfg->x[1];
+    size_t const off_dcos = h.size();
+    for (int j = 0; j < n1; ++j) h.push_back(std::cos(x1[j]) - std::cos(x1[j + 1]));
+    size_t const off_sin = h.size();
+    for (int j = 0; j <= n1; ++j) h.push_back(std::sin(x1[j]));
+    size_t const off_cot = h.size();
+    for (int j = 0; j < n1; ++j) {
+        double const mid = (x1[j] + x1[j + 1]) / 2;
+        h.push_back(std::cos(mid) / std::sin(mid));
+    }
+    size_t const off_sinc = h.size();
+    for (int j = 0; j < n1; ++j) h.push_back(std::sin(h[off_xc[1] + j]));
+    size_t const off_gpm = h.size();
+    for (int i = 0; i < G.nwg[0]; ++i) {
+        double const xc = h[off_xc[0] + i];
+        h.push_back(-kGrav * cfg->central_mass / (xc * xc));
+    }
+
+    auto cleanup = [&](int code) {
+        hclpp_ctx_destroy(c);
+        return code;
+    };
+    hipError_t e = hipGetDevice(&c->device);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipGetDevice"));
+    e = hipMalloc(&c->tables, sizeof(double) * h.size());
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(tables)"));
+    e = hipMemcpy(c->tables, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemcpy(tables)"));
+    for (int d = 0; d < 3; ++d) {
+        G.x[d] = c->tables + off_x[d];
+        G.dx[d] = c->tables + off_dx[d];
+        G.xc[d] = c->tables + off_xc[d];
+    }
+    G.dcos = c->tables + off_dcos;
+    G.sin_node = c->tables + off_sin;
+    G.cot_mid = c->tables + off_cot;
+    G.sin_center = c->tables + off_sinc;
+    G.gpm = c->tables + off_gpm;
+
+    e = hipMalloc(&c->scalars, sizeof(double) * 16);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(scalars)"));
+    e = hipHostMalloc(&c->host_scalars, sizeof(double) * 16, hipHostMallocDefault);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipHostMalloc(scalars)"));
+    e = hipMalloc(&c->clock, sizeof(DevClock));
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(clock)"));
+    e = hipHostMalloc(&c->host_clock, sizeof(DevClock), hipHostMallocDefault);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipHostMalloc(clock)"));
+    std::memset(c->host_clock, 0, sizeof(DevClock));
+    c->host_clock->pub.done = 1;  // no step runs before hclpp_clock_start
+    e = hipMemcpy(c->clock, c->host_clock, sizeof(DevClock), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemcpy(clock)"));
+
+    // fused-step scratch
+    c->shell = make_ghost_shell(G, nd);
+    c->ghost_cap = ghost_blocks(c->shell);
+    if (c->ghost_cap > 0) {
+        e = hipMalloc(&c->ghost_partials, sizeof(double) * c->ghost_cap);
+        if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(ghost partials)"));
+    }
+    c->fused_nb = fused_blocks(G, nd, nullptr);
+    Range3 all;
+    for (int d = 0; d < 3; ++d) {
+        all.b[d] = 0;
+        all.e[d] = G.nwg[d];
+    }
+    int64_t const need = std::max<int64_t>(c->fused_nb * kPartialStride, reduce_blocks(all));
+    int const rc = ensure_partials(c, need);
+    if (rc != HCLPP_OK) return cleanup(rc);
+    *out = c;
+    return HCLPP_OK;
+}
+
+int hclpp_ctx_destroy(hclpp_ctx* c)
+{
+    if (!c) return HCLPP_OK;
+    if (c->tables) (void)hipFree(c->tables);
+    if (c->partials) (void)hipFree(c->partials);
+    if (c->ghost_partials) (void)hipFree(c->ghost_partials);
+    if (c->scalars) (void)hipFree(c->scalars);
+    if (c->host_scalars) (void)hipHostFree(c->host_scalars);
+    if (c->clock) (void)hipFree(c->clock);
+    if (c->host_clock) (void)hipHostFree(c->host_clock);
+    if (c->fx_old) (void)hipFree(c->fx_old);
+    delete c;
+    return HCLPP_OK;
+}
+
+int hclpp_set_stream(hclpp_ctx* c, void* stream)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    c->stream = (hipStream_t)stream;
+    return HCLPP_OK;
+}
+
+int hclpp_range_with_ghosts(const hclpp_ctx* c, int ngeff, hclpp_range* out)
+{
+    if (!c || !out) return fail(HCLPP_ERR_ARG, "null argument");
+    if (ngeff < 0 || ngeff > c->cfg.ng) return fail(HCLPP_ERR_ARG, "NgEff > Ng");  // mesh/range.cpp:99-107
+    for (int d = 0; d < 3; ++d) {
+        int const e = d < c->cfg.ndim ? ngeff : 0;
+        out->begin[d] = c->G.ngh[d] - e;
+        out->end[d] = c->G.ngh[d] + c->G.n[d] + e;
+    }
+    return HCLPP_OK;
+}
+
+int hclpp_conv_prim_to_cons(hclpp_ctx* c, const hclpp_range* r, const double* rho, const double* u, const double* P, double* rhou, double* E)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    int rc = check_range(c, r, 0);
+    if (rc) return rc;
+    launch_prim_to_cons(c->cfg.ndim, c->G, to_range(r), c->eos, rho, u, P, rhou, E, c->stream);
+    HCLPP_LAUNCHED("conv_prim_to_cons");
+    return HCLPP_OK;
+}
+
+int hclpp_conv_cons_to_prim(hclpp_ctx* c, const hclpp_range* r, const double* rho, const double* rhou, const double* E, double* u, double* P)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    int rc = check_range(c, r, 0);
+    if (rc) return rc;
+    launch_cons_to_prim(c->cfg.ndim, c->G, to_range(r), c->eos, rho, rhou, E, u, P, c->stream);
+    HCLPP_LAUNCHED("conv_cons_to_prim");
+    return HCLPP_OK;
+}
+
+int hclpp_face_reconstruction(hclpp_ctx* c, int limiter, const hclpp_range* r, const double* var, double* var_rec)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (limiter < 0 || limiter > 3) return fail(HCLPP_ERR_ARG, "Unknown face reconstruction algorithm");
+    int rc = check_range(c, r, 1);
+    if (rc) return rc;
+    launch_face_reconstruction(c->cfg.ndim, limiter, c->G, to_range(r), var, var_rec, c->stream);
+    HCLPP_LAUNCHED("face_reconstruction");
+    return HCLPP_OK;
+}
+
+static int hancock_impl(hclpp_ctx* c, const hclpp_range* r, double dt, const double* u_rec, const double* P_rec, double* rho_rec,
+                        double* rhou_rec, double* E_rec, double* fx_rec)
+{
+    int const nd = c->cfg.ndim, nfx = c->cfg.nfx;
+    if (nfx > 0) {  // extrapolation_reconstruction.hpp:87-88: the mass fractions before the update
+        int64_t const n = c->G.plane * 2 * nd * nfx;
+        if (n > c->fx_old_cap) {
+            if (c->fx_old) HCLPP_HIP(hipFree(c->fx_old));
+            c->fx_old = nullptr;
+            HCLPP_HIP(hipMalloc(&c->fx_old, sizeof(double) * n));
+            c->fx_old_cap = n;
+        }
+        HCLPP_HIP(hipMemcpyAsync(c->fx_old, fx_rec, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    launch_hancock(nd, c->cfg.geom, c->G, to_range(r), c->eos, dt, u_rec, P_rec, rho_rec, rhou_rec, E_rec, fx_rec, c->fx_old, nfx, c->stream);
+    HCLPP_LAUNCHED("Hancock_extrapolation");
+    return HCLPP_OK;
+}
+
+int hclpp_hancock_extrapolation(hclpp_ctx* c, const hclpp_range* r, double dt, const double* u_rec, const double* P_rec, double* rho_rec,
+                                double* rhou_rec, double* E_rec, double* fx_rec)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    int rc = check_range(c, r, 1);
+    if (rc) return rc;
+    return hancock_impl(c, r, dt, u_rec, P_rec, rho_rec, rhou_rec, E_rec, fx_rec);
+}
+
+int hclpp_hydro_reconstruction(hclpp_ctx* c, int limiter, const hclpp_range* r, double dt, const double* rho, const double* u, const double* P,
+                               const double* fx, double* rho_rec, double* rhou_rec, double* E_rec, double* fx_rec, double* P_rec,
+                               double* u_rec)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (limiter < 0 || limiter > 3) return fail(HCLPP_ERR_ARG, "Unknown face reconstruction algorithm");
+    int rc = check_range(c, r, 1);
+    if (rc) return rc;
+    int const nd = c->cfg.ndim, nfx = c->cfg.nfx;
+    int64_t const pl = c->G.plane;
+    Range3 const R = to_range(r);
+    // hydro_reconstruction.hpp:99-109
+    launch_face_reconstruction(nd, limiter, c->G, R, rho, rho_rec, c->stream);
+    for (int d = 0; d < nd; ++d) launch_face_reconstruction(nd, limiter, c->G, R, u + pl * d, u_rec + pl * 2 * nd * d, c->stream);
+    launch_face_reconstruction(nd, limiter, c->G, R, P, P_rec, c->stream);
+    for (int f = 0; f < nfx; ++f) launch_face_reconstruction(nd, limiter, c->G, R, fx + pl * f, fx_rec + pl * 2 * nd * f, c->stream);
+    // :111-126
+    launch_face_prim_to_cons(nd, c->G, R, c->eos, rho_rec, u_rec, P_rec, rhou_rec, E_rec, c->stream);
+    HCLPP_LAUNCHED("face_reconstruction / conv_prim_to_cons");
+    // :128
+    return hancock_impl(c, r, dt, u_rec, P_rec, rho_rec, rhou_rec, E_rec, fx_rec);
+}
+
+int hclpp_godunov(hclpp_ctx* c, int riemann, const hclpp_range* r, double dt, const double* rho, const double* rhou, const double* E,
+                  const double* fx, const double* rho_rec, const double* rhou_rec, const double* E_rec, const double* fx_rec, double* rho_new,
+                  double* rhou_new, double* E_new, double* fx_new)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (riemann < 0 || riemann > 2) return fail(HCLPP_ERR_ARG, "Invalid riemann solver");
+    int rc = check_range(c, r, 1);
+    if (rc) return rc;
+    launch_godunov(c->cfg.ndim, c->cfg.geom, riemann, c->G, to_range(r), c->eos, dt, rho, rhou, E, fx, rho_rec, rhou_rec, E_rec, fx_rec, rho_new,
+                   rhou_new, E_new, fx_new, c->cfg.nfx, c->stream);
+    HCLPP_LAUNCHED("Riemann_based_Godunov_scheme");
+    return HCLPP_OK;
+}
+
+int hclpp_time_step(hclpp_ctx* c, const hclpp_range* r, const double* rho, const double* u, const double* P, double* dt_out)
+{
+    if (!c || !dt_out) return fail(HCLPP_ERR_ARG, "null argument");
+    int rc = check_range(c, r, 0);
+    if (rc) return rc;
+    Range3 const R = to_range(r);
+    rc = ensure_partials(c, reduce_blocks(R));
+    if (rc) return rc;
+    launch_time_step_rate(c->cfg.ndim, c->cfg.geom, c->G, R, c->eos, rho, u, P, c->partials, c->scalars, c->stream);
+    HCLPP_LAUNCHED("time_step");
+    double inv;
+    rc = read_scalar(c, &inv);
+    if (rc) return rc;
+    *dt_out = 1 / inv;  // time_step.hpp:81
+    return HCLPP_OK;
+}
+
+int hclpp_min_internal_energy(hclpp_ctx* c, const hclpp_range* r, const double* rho, const double* rhou, const double* E, double* out)
+{
+    if (!c || !out) return fail(HCLPP_ERR_ARG, "null argument");
+    int rc = check_range(c, r, 0);
+    if (rc) return rc;
+    Range3 const R = to_range(r);
+    rc = ensure_partials(c, reduce_blocks(R));
+    if (rc) return rc;
+    launch_min_internal_energy(c->cfg.ndim, c->G, R, rho, rhou, E, c->partials, c->scalars, c->stream);
+    HCLPP_LAUNCHED("min_internal_energy");
+    return read_scalar(c, out);
+}
+
+int hclpp_integrate(hclpp_ctx* c, const hclpp_range* r, const double* var, double* out)
+{
+    if (!c || !out) return fail(HCLPP_ERR_ARG, "null argument");
+    int rc = check_range(c, r, 0);
+    if (rc) return rc;
+    Range3 const R = to_range(r);
+    rc = ensure_partials(c, reduce_blocks(R));
+    if (rc) return rc;
+    launch_integrate(c->G, c->cfg.ndim, c->cfg.geom, R, var, c->partials, c->scalars, c->stream);
+    HCLPP_LAUNCHED("integration");
+    return read_scalar(c, out);
+}
+
+int hclpp_fill_ghosts(hclpp_ctx* c, double* rho, double* rhou, double* E, double* fx)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    launch_fill_ghosts(c->cfg.ndim, c->G, c->bc, rho, rhou, E, fx, c->cfg.nfx, c->stream);
+    HCLPP_LAUNCHED("boundary conditions");
+    return HCLPP_OK;
+}
+
+int hclpp_clock_start(hclpp_ctx* c, double t0, double t_end, double cfl, int max_iter, const double* rho, const double* rhou, const double* E)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (!(cfl > 0) || max_iter < 1) return fail(HCLPP_ERR_ARG, "cfl must be > 0 and max_iter >= 1");
+    DevClock* h = c->host_clock;
+    HCLPP_HIP(hipStreamSynchronize(c->stream));  // the pinned staging buffer is reused
+    std::memset(h, 0, sizeof(DevClock));
+    h->pub.t = t0;
+    h->pub.t_end = t_end;
+    h->pub.cfl = cfl;
+    h->pub.max_iter = max_iter;
+    h->pub.min_eint = NAN;
+    HCLPP_HIP(hipMemcpyAsync(c->clock, h, sizeof(DevClock), hipMemcpyHostToDevice, c->stream));
+    Range3 all;
+    for (int d = 0; d < 3; ++d) {
+        all.b[d] = 0;
+        all.e[d] = c->G.nwg[d];
+    }
+    int rc = ensure_partials(c, reduce_blocks(all));
+    if (rc) return rc;
+    // main.cpp:355: dt from the ghost-inclusive state
+    launch_time_step_cons(c->cfg.ndim, c->cfg.geom, c->G, all, c->eos, rho, rhou, E, c->partials, c->scalars, c->stream);
+    launch_clock_start(c->clock, c->scalars, c->stream);
+    HCLPP_LAUNCHED("clock start");
+    HCLPP_HIP(hipStreamSynchronize(c->stream));
+    return HCLPP_OK;
+}
+
+int hclpp_clock_read(hclpp_ctx* c, hclpp_clock* out)
+{
+    if (!c || !out) return fail(HCLPP_ERR_ARG, "null argument");
+    HCLPP_HIP(hipMemcpyAsync(c->host_clock, c->clock, sizeof(DevClock), hipMemcpyDeviceToHost, c->stream));
+    HCLPP_HIP(hipStreamSynchronize(c->stream));
+    *out = c->host_clock->pub;
+    return HCLPP_OK;
+}
+
+int hclpp_step(hclpp_ctx* c, const double* rho, const double* rhou, const double* E, double* rho_new, double* rhou_new, double* E_new)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (c->cfg.nfx > 0) return fail(HCLPP_ERR_UNSUPPORTED, "the fused step has no passive scalars; use the operator entry points");
+    if (c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "hclpp_step takes FP64 storage");
+    if (!rho || !rhou || !E || !rho_new || !rhou_new || !E_new) return fail(HCLPP_ERR_ARG, "null array");
+    if (rho == rho_new || E == E_new || rhou == rhou_new) return fail(HCLPP_ERR_ARG, "input and output arrays must differ");
+    StepArrays A;
+    A.rho = rho;
+    A.rhou = rhou;
+    A.E = E;
+    A.rho_new = rho_new;
+    A.rhou_new = rhou_new;
+    A.E_new = E_new;
+    A.partials = c->partials;
+    A.ghost_partials = c->ghost_partials;
+    A.clock = c->clock;
+    int const nd = c->cfg.ndim;
+    launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->eos, A, c->stream);
+    HCLPP_LAUNCHED("fused step");
+    launch_ghosts_rate(nd, c->cfg.geom, c->G, c->eos, c->bc, c->shell, A, c->stream);
+    HCLPP_LAUNCHED("ghost fill");
+    launch_clock_advance(nd, c->clock, c->partials, c->fused_nb, c->ghost_partials, c->ghost_cap, c->stream);
+    HCLPP_LAUNCHED("clock advance");
+    return HCLPP_OK;
+}
+
+}  // extern "C"

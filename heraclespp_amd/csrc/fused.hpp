@@ -1,0 +1,76 @@
+// fused.hpp -- structures shared by the fused step kernels and the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "grid.hpp"
+#include "hclpp_hip.h"
+#include "launch.hpp"
+
+namespace hclpp {
+
+// Device-resident clock: the public hclpp_clock plus the exit flag of the step in flight.
+struct DevClock {
+    hclpp_clock pub;
+    int pending_exit;
+};
+
+// Per-workgroup partials of the fused step: [0] max CFL rate, [1] min internal
+// energy, [2..2+NV) sums of the conserved variables times dV.
+constexpr int kPartialStride = 8;
+
+struct StepArrays {
+    const double* __restrict__ rho;
+    const double* __restrict__ rhou;
+    const double* __restrict__ E;
+    double* __restrict__ rho_new;
+    double* __restrict__ rhou_new;
+    double* __restrict__ E_new;
+    double* __restrict__ partials;
+    double* __restrict__ ghost_partials;
+    DevClock* clock;
+};
+
+struct FusedLayout {
+    int ntile[2];  // in-plane tile counts
+    int seglen;    // planes per workgroup along the march dim
+};
+
+// Ghost shell split into 2*ndim disjoint boxes (slab d: ghost along d,
+// interior along later dims, any index along earlier dims).
+struct GhostShell {
+    int nbox;
+    int lo[6][3];
+    int hi[6][3];
+    int64_t start[7];
+};
+
+// Workgroup shape of the fused kernel per dimensionality.
+template <int ND>
+struct MarchShape;
+template <>
+struct MarchShape<1> {  // 64 independent segments of 16 cells per workgroup
+    static constexpr int HX = 1, HY = 1, NT = 64, SEG = 16;
+};
+template <>
+struct MarchShape<2> {  // a 64-column row (62 outputs + halo) marching along y
+    static constexpr int HX = 64, HY = 1, NT = 64, SEG = 32;
+};
+template <>
+struct MarchShape<3> {  // 32 x 8 columns (30 x 6 outputs + halo) marching along z
+    static constexpr int HX = 32, HY = 8, NT = 256, SEG = 64;
+};
+
+int64_t fused_blocks(const GridDev& G, int nd, FusedLayout* layout);
+void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, hipStream_t s);
+GhostShell make_ghost_shell(const GridDev& G, int nd);
+int64_t ghost_blocks(const GhostShell& B);
+void launch_ghosts_rate(int nd, int geom, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const GhostShell& B, const StepArrays& A,
+                        hipStream_t s);
+void launch_clock_start(DevClock* c, const double* inv, hipStream_t s);
+void launch_clock_advance(int nd, DevClock* c, const double* partials, int64_t nblocks, const double* ghost_partials, int64_t nghost,
+                          hipStream_t s);
+
+}  // namespace hclpp

@@ -1,0 +1,235 @@
+"""Time-loop driver: the reference main.cpp:305-311 (initialisation) and
+:354-531 (loop body) on one GPU.
+
+Two paths, both HIP:
+  * fused (default): one hclpp_step per iteration -- the fused
+    reconstruct/predict/Riemann/update kernel, the ghost fill and the device
+    clock (dt, t_end clipping, max_iter, the internal-energy check) -- with no
+    host synchronisation inside the loop;
+  * operators: the reference operator sequence (time_step,
+    minimum_internal_energy, MUSCL-Hancock reconstruction, Godunov update,
+    boundaries, cons->prim), one kernel per reference kernel, host-synchronised
+    on dt exactly like the reference.  Needed for passive scalars and used to
+    bisect parity.
+Time outputs are not modelled (the reference clips dt to output times only
+when an output frequency is configured).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .hydro import (Context, Grid, PerfectGas, PointMassGravity, Range, UniformGravity)
+
+
+def _dev(a, shape, device="cuda"):
+    if isinstance(a, np.ndarray):
+        a = torch.from_numpy(np.ascontiguousarray(a))
+    a = a.to(device=device, dtype=torch.float64).contiguous()
+    if tuple(a.shape) != tuple(shape):
+        raise ValueError(f"expected shape {tuple(shape)}, got {tuple(a.shape)}")
+    return a
+
+
+class HydroSolver:
+    def __init__(self, grid: Grid, eos: PerfectGas, *, reconstruction: str = "VanLeer", riemann_solver: str = "HLL",
+                 gravity=None, bc=None, bc_order=None, cfl: float = 0.4, nfx: int = 0, fused: bool = True):
+        self.grid = grid
+        self.eos = eos
+        self.gravity = gravity
+        self.cfl = float(cfl)
+        self.nfx = nfx
+        self.fused = fused and nfx == 0
+        self.ctx = Context(grid, eos, gravity, bc, bc_order, nfx, reconstruction, riemann_solver)
+        nd = grid.ndim
+        z = grid.zeros
+        self._cons = [(z(), z(nd), z()), (z(), z(nd), z())]
+        self._cur = 0
+        self.fx = z(max(nfx, 0)) if nfx else None
+        self.u = z(nd)
+        self.P = z()
+        self.t = 0.0
+        self.iter = 0
+        self._clock_started = False
+        if not self.fused:
+            self.rho_rec = z(nd, 2)
+            self.rhou_rec = z(nd, nd, 2)
+            self.E_rec = z(nd, 2)
+            self.u_rec = z(nd, nd, 2)
+            self.P_rec = z(nd, 2)
+            self.fx_rec = z(nfx, nd, 2) if nfx else None
+            self.fx_new = z(nfx) if nfx else None
+
+    # ---- state access
+    @property
+    def rho(self):
+        return self._cons[self._cur][0]
+
+    @property
+    def rhou(self):
+        return self._cons[self._cur][1]
+
+    @property
+    def E(self):
+        return self._cons[self._cur][2]
+
+    def _r(self, ngeff) -> ctypes.Structure:
+        return self.grid.range.with_ghosts(ngeff)._c()
+
+    def update_primitives(self):
+        """conv_cons_to_prim over all ghosts (main.cpp:311, :526)."""
+        c = self.ctx
+        check(lib().hclpp_conv_cons_to_prim(c.handle, ctypes.byref(self._r(self.grid.Ng)), c._chk(self.rho),
+                                            c._chk(self.rhou, (self.grid.ndim,)), c._chk(self.E),
+                                            c._chk(self.u, (self.grid.ndim,)), c._chk(self.P)))
+
+    # ---- main.cpp:305-311
+    def init_from_prim(self, rho, u, P, fx=None, t0: float = 0.0):
+        g = self.grid
+        nd = g.ndim
+        rho_d = _dev(rho, g.shape3)
+        self.u.copy_(_dev(u, (nd,) + g.shape3))
+        self.P.copy_(_dev(P, g.shape3))
+        self._cur = 0
+        r, m, E = self._cons[0]
+        r.copy_(rho_d)
+        if self.nfx:
+            self.fx.copy_(_dev(fx, (self.nfx,) + g.shape3))
+        c = self.ctx
+        check(lib().hclpp_conv_prim_to_cons(c.handle, ctypes.byref(self._r(0)), c._chk(r), c._chk(self.u, (nd,)),
+                                            c._chk(self.P), c._chk(m, (nd,)), c._chk(E)))
+        check(lib().hclpp_fill_ghosts(c.handle, c._chk(r), c._chk(m, (nd,)), c._chk(E),
+                                      c._fx(self.fx, (self.nfx,), "fx")))
+        self.update_primitives()
+        self.t = float(t0)
+        self.iter = 0
+        self._clock_started = False
+
+    # ---- main.cpp:354-531
+    def run(self, t_end: float, max_iter: int, check_every: int = 64) -> int:
+        """Advance until t_end or max_iter (as the reference loop); returns the number of steps taken."""
+        if self.fused:
+            return self._run_fused(t_end, max_iter, check_every)
+        n = 0
+        while True:
+            done = self._step_operators(t_end, max_iter)
+            n += 1
+            if done:
+                return n
+
+    def _run_fused(self, t_end, max_iter, check_every):
+        c = self.ctx
+        nd = self.grid.ndim
+        it0 = self.iter
+        cur0 = self._cur
+        r, m, E = self._cons[self._cur]
+        check(lib().hclpp_clock_start(c.handle, self.t, float(t_end), self.cfl, int(max_iter), c._chk(r),
+                                      c._chk(m, (nd,)), c._chk(E)))
+        launched = 0
+        while True:
+            batch = check_every
+            if max_iter - it0 - launched > 0:
+                batch = min(batch, max_iter - it0 - launched)
+            for _ in range(batch):
+                a = self._cons[(cur0 + launched) % 2]
+                b = self._cons[(cur0 + launched + 1) % 2]
+                check(lib().hclpp_step(c.handle, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(b[0]), _ptr(b[1]), _ptr(b[2])))
+                launched += 1
+            clk = self.clock()
+            if clk.done or clk.status:
+                break
+        steps = clk.iter - it0
+        self._cur = (cur0 + steps) % 2
+        self.t = clk.t
+        self.iter = clk.iter
+        self.update_primitives()
+        if clk.status:
+            raise RuntimeError(f"Time = {clk.t}, iteration = {clk.iter}: detected invalid volumic internal energy")
+        return steps
+
+    def start_clock(self, t_end: float, max_iter: int):
+        """Start the device clock on the current state (for callers that drive step_async themselves)."""
+        c = self.ctx
+        nd = self.grid.ndim
+        r, m, E = self._cons[self._cur]
+        check(lib().hclpp_clock_start(c.handle, self.t, float(t_end), self.cfl, int(max_iter), c._chk(r),
+                                      c._chk(m, (nd,)), c._chk(E)))
+        self._launched = 0
+        self._cur0 = self._cur
+        self._it0 = self.iter
+
+    def step_async(self):
+        """Enqueue one fused step (no host synchronisation)."""
+        a = self._cons[(self._cur0 + self._launched) % 2]
+        b = self._cons[(self._cur0 + self._launched + 1) % 2]
+        check(lib().hclpp_step(self.ctx.handle, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(b[0]), _ptr(b[1]), _ptr(b[2])))
+        self._launched += 1
+
+    def finish(self):
+        """Synchronise with the device clock after step_async calls."""
+        clk = self.clock()
+        steps = clk.iter - self._it0
+        self._cur = (self._cur0 + steps) % 2
+        self.t = clk.t
+        self.iter = clk.iter
+        if clk.status:
+            raise RuntimeError(f"Time = {clk.t}, iteration = {clk.iter}: detected invalid volumic internal energy")
+        return clk
+
+    def clock(self) -> _lib.Clock:
+        out = _lib.Clock()
+        check(lib().hclpp_clock_read(self.ctx.handle, ctypes.byref(out)))
+        return out
+
+    def _step_operators(self, t_end, max_iter) -> bool:
+        g = self.grid
+        c = self.ctx
+        nd = g.ndim
+        L = lib()
+        h = c.handle
+        inv = ctypes.c_double()
+        check(L.hclpp_time_step(h, ctypes.byref(self._r(g.Ng)), c._chk(self.rho), c._chk(self.u, (nd,)),
+                                c._chk(self.P), ctypes.byref(inv)))
+        dt = self.cfl * inv.value  # main.cpp:355
+        exit_ = False
+        if (self.t + dt) >= t_end:  # :377-381
+            dt = t_end - self.t
+            exit_ = True
+        if (self.iter + 1) >= max_iter:  # :382-385
+            exit_ = True
+        me = ctypes.c_double()
+        check(L.hclpp_min_internal_energy(h, ctypes.byref(self._r(0)), c._chk(self.rho), c._chk(self.rhou, (nd,)),
+                                          c._chk(self.E), ctypes.byref(me)))
+        if np.isnan(me.value) or me.value < 0:  # :397-425
+            raise RuntimeError(f"Time = {self.t}, iteration = {self.iter}: detected invalid volumic internal energy")
+        nfx = self.nfx
+        fx = c._fx(self.fx, (nfx,), "fx")
+        check(L.hclpp_hydro_reconstruction(h, _lib.LIMITER[c.limiter], ctypes.byref(self._r(1)), dt / 2,
+                                           c._chk(self.rho), c._chk(self.u, (nd,)), c._chk(self.P), fx,
+                                           c._chk(self.rho_rec, (nd, 2)), c._chk(self.rhou_rec, (nd, nd, 2)),
+                                           c._chk(self.E_rec, (nd, 2)), c._fx(self.fx_rec, (nfx, nd, 2), "fx_rec"),
+                                           c._chk(self.P_rec, (nd, 2)), c._chk(self.u_rec, (nd, nd, 2))))
+        nxt = self._cons[1 - self._cur]
+        check(L.hclpp_godunov(h, _lib.RIEMANN[c.riemann], ctypes.byref(self._r(0)), dt, c._chk(self.rho),
+                              c._chk(self.rhou, (nd,)), c._chk(self.E), fx, c._chk(self.rho_rec, (nd, 2)),
+                              c._chk(self.rhou_rec, (nd, nd, 2)), c._chk(self.E_rec, (nd, 2)),
+                              c._fx(self.fx_rec, (nfx, nd, 2), "fx_rec"), c._chk(nxt[0]), c._chk(nxt[1], (nd,)),
+                              c._chk(nxt[2]), c._fx(self.fx_new, (nfx,), "fx_new")))
+        check(L.hclpp_fill_ghosts(h, c._chk(nxt[0]), c._chk(nxt[1], (nd,)), c._chk(nxt[2]),
+                                  c._fx(self.fx_new, (nfx,), "fx_new")))
+        self._cur = 1 - self._cur
+        if nfx:
+            self.fx, self.fx_new = self.fx_new, self.fx
+        self.update_primitives()
+        self.t += dt
+        self.iter += 1
+        return exit_
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())

@@ -1,0 +1,167 @@
+/*
+ * hclpp_hip.h -- C-ABI of libhclpp_hip.so, the MI355X (gfx950) HIP
+ * implementation of the HERACLES++ hydro step.
+ *
+ * Plain C: opaque context, plain device pointers and sizes, int status codes
+ * (HCLPP_OK = 0) with a thread-local message from hclpp_last_error().  No
+ * exception crosses this boundary.
+ *
+ * Data layout (identical to the reference's Kokkos::LayoutLeft views, so a
+ * caller hands over View::data() unchanged): every array spans the
+ * ghost-inclusive extents nwg[0] x nwg[1] x nwg[2] (nwg[d] = n[d] + 2 ng on
+ * active dims, n[d] = 1 on inactive dims), flat index
+ * i + nwg0 * (j + nwg1 * k), and component indices are slowest:
+ *   u, rhou          [wg, ndim]                 (component plane stride = nwg0*nwg1*nwg2)
+ *   fx               [wg, nfx]
+ *   var_rec, P_rec   [wg, side(2), idim(ndim)]
+ *   rho_rec, E_rec   [wg, side(2), idim(ndim)]
+ *   u_rec, rhou_rec  [wg, side(2), idim(ndim), component(ndim)]
+ *   fx_rec           [wg, side(2), idim(ndim), nfx]
+ * All array pointers are device pointers.  Work is enqueued on the
+ * context's stream (hclpp_set_stream); functions that return a host scalar
+ * synchronise that stream.
+ *
+ * Citations are to the reference tree (src/...).
+ */
+#ifndef HCLPP_HIP_H
+#define HCLPP_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HCLPP_OK 0
+#define HCLPP_ERR_ARG 1      /* invalid argument or configuration */
+#define HCLPP_ERR_HIP 2      /* HIP runtime error */
+#define HCLPP_ERR_STATE 3    /* invalid physical state detected */
+#define HCLPP_ERR_UNSUPPORTED 4
+
+enum { HCLPP_GEOM_CARTESIAN = 0, HCLPP_GEOM_SPHERICAL = 1 };
+enum { HCLPP_LIM_CONSTANT = 0, HCLPP_LIM_VANLEER = 1, HCLPP_LIM_MINMOD = 2, HCLPP_LIM_VANALBADA = 3 };
+enum { HCLPP_RS_HLL = 0, HCLPP_RS_HLLC = 1, HCLPP_RS_LOWMACH = 2 };
+enum { HCLPP_GRAV_UNIFORM = 0, HCLPP_GRAV_POINT_MASS = 1 };
+enum { HCLPP_BC_PERIODIC = 0, HCLPP_BC_NULLGRADIENT = 1, HCLPP_BC_REFLEXIVE = 2 };
+enum { HCLPP_FP64 = 0, HCLPP_FP32_STORAGE = 1 };
+
+typedef struct hclpp_ctx hclpp_ctx;
+
+/* Replaces the compile-time ndim/geom/EoS/gravity choices and the run-time
+ * Grid, PerfectGas, gravity and boundary objects built in main.cpp:140-303
+ * (Grid: mesh/grid.cpp:48-76; PerfectGas(gamma, mmw): perfect_gas.cpp:13-22;
+ * make_uniform_gravity / make_point_mass_gravity: gravity/gravity.cpp:29-55;
+ * bc choices and priority: main.cpp:175-186, boundary_distribute.cpp:33-71). */
+typedef struct {
+    int ndim;          /* 1, 2 or 3 */
+    int geom;          /* HCLPP_GEOM_* */
+    int limiter;       /* HCLPP_LIM_*: used by the fused step */
+    int riemann;       /* HCLPP_RS_*: used by the fused step */
+    int gravity;       /* HCLPP_GRAV_* */
+    int nfx;           /* number of passive scalars */
+    int ng;            /* ghost layers on active dims (the reference default is 2) */
+    int n[3];          /* interior cells per dim (1 on inactive dims) */
+    int bc[6];         /* per face 2*dim+side: HCLPP_BC_* */
+    int bc_order[6];   /* order of border BCs (0..2*ndim-1) */
+    double gamma;      /* adiabatic index (> 1) */
+    double mmw;        /* mean molecular weight (> 0) */
+    double g[3];       /* uniform gravity */
+    double central_mass; /* point-mass gravity */
+    const double* x[3];  /* HOST node coordinates, nwg[d] + 1 entries (ghosts included) */
+    int precision;     /* HCLPP_FP64 or HCLPP_FP32_STORAGE (fused step only) */
+} hclpp_config;
+
+/* Cell range [begin, end) per dim in ghost-inclusive indices (mesh/range.hpp:19-55). */
+typedef struct {
+    int begin[3];
+    int end[3];
+} hclpp_range;
+
+const char* hclpp_last_error(void);
+int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out);
+int hclpp_ctx_destroy(hclpp_ctx* ctx);
+/* stream is a hipStream_t (NULL: the null stream). */
+int hclpp_set_stream(hclpp_ctx* ctx, void* stream);
+/* Range::with_ghosts(ngeff) of the whole local domain (mesh/range.cpp:89-107, :120-135). */
+int hclpp_range_with_ghosts(const hclpp_ctx* ctx, int ngeff, hclpp_range* out);
+/* name -> enum as the reference factories map them; -1 if unknown
+ * (face_reconstruction.cpp:31-50, godunov_scheme.hpp:305-321, boundary_factory.hpp:31-47). */
+int hclpp_limiter_from_name(const char* name);
+int hclpp_riemann_from_name(const char* name);
+int hclpp_bc_from_name(const char* name);
+
+/* ---- reference operator interfaces (FP64), one kernel per reference kernel ---- */
+
+/* conv_prim_to_cons: hydro/array_conversion.hpp:32-97 */
+int hclpp_conv_prim_to_cons(hclpp_ctx* ctx, const hclpp_range* range, const double* rho, const double* u, const double* P,
+                            double* rhou, double* E);
+/* conv_cons_to_prim: hydro/array_conversion.hpp:107-172 */
+int hclpp_conv_cons_to_prim(hclpp_ctx* ctx, const hclpp_range* range, const double* rho, const double* rhou, const double* E,
+                            double* u, double* P);
+/* IFaceReconstruction::execute with the LimitedLinearReconstruction<limiter> made by
+ * factory_face_reconstruction: hydro/face_reconstruction.hpp:21-43,
+ * limited_linear_reconstruction.hpp:34-65 */
+int hclpp_face_reconstruction(hclpp_ctx* ctx, int limiter, const hclpp_range* range, const double* var, double* var_rec);
+/* IExtrapolationReconstruction::execute of HancockExtrapolationReconstruction:
+ * hydro/extrapolation_reconstruction.hpp:27-54, :65-338.  dt is the half step. */
+int hclpp_hancock_extrapolation(hclpp_ctx* ctx, const hclpp_range* range, double dt, const double* u_rec, const double* P_rec,
+                                double* rho_rec, double* rhou_rec, double* E_rec, double* fx_rec);
+/* IHydroReconstruction::execute of MUSCLHancockHydroReconstruction (P_rec, u_rec are
+ * its scratch views): hydro/hydro_reconstruction.hpp:30-59, :85-129 */
+int hclpp_hydro_reconstruction(hclpp_ctx* ctx, int limiter, const hclpp_range* range, double dt, const double* rho, const double* u,
+                               const double* P, const double* fx, double* rho_rec, double* rhou_rec, double* E_rec, double* fx_rec,
+                               double* P_rec, double* u_rec);
+/* IGodunovScheme::execute of RiemannBasedGodunovScheme<solver> made by
+ * factory_godunov_scheme: hydro/godunov_scheme.hpp:31-64, :77-302, :305-321 */
+int hclpp_godunov(hclpp_ctx* ctx, int riemann, const hclpp_range* range, double dt, const double* rho, const double* rhou,
+                  const double* E, const double* fx, const double* rho_rec, const double* rhou_rec, const double* E_rec,
+                  const double* fx_rec, double* rho_new, double* rhou_new, double* E_new, double* fx_new);
+/* time_step<EoS>: hydro/time_step.hpp:35-82; *dt_out = 1 / max(sum_d (|u_d|+c)/dx_geom_d)
+ * (the caller multiplies by cfl, as main.cpp:355 does).  Synchronises. */
+int hclpp_time_step(hclpp_ctx* ctx, const hclpp_range* range, const double* rho, const double* u, const double* P, double* dt_out);
+/* minimum_internal_energy: analysis/internal_energy.cpp:20-45.  Synchronises. */
+int hclpp_min_internal_energy(hclpp_ctx* ctx, const hclpp_range* range, const double* rho, const double* rhou, const double* E,
+                              double* out);
+/* integrate: analysis/integration.cpp:22-37 (sum of var * dV).  Synchronises. */
+int hclpp_integrate(hclpp_ctx* ctx, const hclpp_range* range, const double* var, double* out);
+/* DistributedBoundaryCondition::operator() on one rank: boundary_distribute.hpp:42-76,
+ * boundary_distribute.cpp:75-120, boundary.cpp:33-108 (conservative arrays). */
+int hclpp_fill_ghosts(hclpp_ctx* ctx, double* rho, double* rhou, double* E, double* fx);
+
+/* ---- fused step with a device-resident clock (replaces the loop body of main.cpp:354-531) ----
+ * The clock lives in device memory so consecutive steps run without host
+ * synchronisation: each step reads dt from the clock, and its epilogue
+ * computes the next dt = cfl * (1 / max CFL rate over all cells including
+ * ghosts), clips it to t_end, counts the iteration and raises `done` when
+ * t_end or max_iter is reached (main.cpp:355-385).  A step launched after
+ * `done` is a no-op.  min_eint holds the minimum internal energy of the state
+ * each step consumed (main.cpp:397-425); status becomes HCLPP_ERR_STATE when it
+ * was NaN or negative, and every later step is a no-op. */
+typedef struct {
+    double t;        /* time of the current state */
+    double dt;       /* step the next call will take */
+    double t_end;
+    double cfl;
+    double min_eint; /* minimum internal energy seen by the last step (interior) */
+    double mass;     /* integrals over the interior of the last state produced */
+    double momentum[3];
+    double energy;
+    int iter;
+    int max_iter;
+    int done;
+    int status;
+} hclpp_clock;
+
+/* Start the clock on a ghost-filled conservative state (FP64 arrays). */
+int hclpp_clock_start(hclpp_ctx* ctx, double t0, double t_end, double cfl, int max_iter, const double* rho, const double* rhou,
+                      const double* E);
+/* Copy the device clock to the host (synchronises). */
+int hclpp_clock_read(hclpp_ctx* ctx, hclpp_clock* out);
+/* One fused step: reads (rho, rhou, E) ghost-filled, writes the new state,
+ * ghosts included, to the *_new arrays (which must not alias the inputs).
+ * FP64 storage; HCLPP_ERR_UNSUPPORTED when the context has nfx > 0. */
+int hclpp_step(hclpp_ctx* ctx, const double* rho, const double* rhou, const double* E, double* rho_new, double* rhou_new,
+               double* E_new);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
