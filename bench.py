@@ -1,0 +1,214 @@
+"""Benchmark of the HERACLES++ hydro step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kelvin_helmholtz]
+
+A step is one fused MUSCL-Hancock + Riemann + Godunov pass over the whole
+grid plus the ghost fill and the device-side dt reduction (hclpp_step).  The
+default workload is the 2D Kelvin-Helmholtz configuration (2048 x 2048,
+periodic, VanLeer + HLLC, gamma 1.4, FP64) on synthetic initial data.  The
+state is resident in HBM before the timed region; W untimed steps run
+first, then exactly K steps are timed between torch.cuda.synchronize()
+calls.  Prints one JSON line:
+  value      Gcell-updates/s = interior cells x K / wall seconds
+  roofline   the fused kernel's algorithmic HBM bytes (read + write the
+             conservative state once: 2 (ndim + 2) x 8 B per cell) divided by
+             its mean duration, measured with HIP events on its stream,
+             against the 8 TB/s HBM3E peak; `traffic` is the measured
+             HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled
+             by a copy-kernel calibration) read from profiles/, or null
+  cpu_baseline  the CPU oracle (a restatement of the reference's unfused
+             kernels, OpenMP) on a bounded sample of the same workload, timed
+             on this host's cores
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gcell-updates/s & HBM roofline %, FP64 vs mixed FP32; adjoint/forward ratio"
+
+WORKLOADS = {
+    # name: (setup factory kwargs at full size, cpu-baseline sample kwargs, description)
+    "kelvin_helmholtz": ({"n": 2048}, {"n": 2048}, "2D Kelvin-Helmholtz 2048x2048 periodic, VanLeer+HLLC"),
+    "rayleigh_taylor": ({"n": 512}, {"n": 128}, "3D Rayleigh-Taylor 512^3, g_z=-0.1, Minmod+HLLC"),
+    "sedov": ({"nr": 512, "nth": 128, "nph": 256}, {"nr": 128, "nth": 32, "nph": 64},
+              "3D spherical Sedov 512x128x256, HLL+VanLeer"),
+    "smooth_euler": ({"n": 1024}, {"n": 1024}, "2D smooth periodic Euler 1024x1024, VanAlbada+HLLC"),
+    "sod": ({"n": 1000}, {"n": 1000}, "1D Sod 1000 cells, Minmod+HLL"),
+}
+
+
+def _setup(name, kw):
+    from heraclespp_amd import setups
+
+    return setups.CONFIGS[name](**kw)
+
+
+def _traffic(workload: str):
+    """Measured HBM bytes per fused-kernel launch from the committed rocprofv3 summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"k_step_march_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(name: str, seconds: float):
+    """Time the oracle (reference kernel sequence on the CPU) on a bounded sample of the workload."""
+    from oracle import oracle as orc
+
+    kw = WORKLOADS[name][1]
+    s = _setup(name, kw)
+    o = orc.Oracle(s.ndim, s.n, geom=s.geom, limiter=s.reconstruction, riemann=s.riemann_solver, gamma=s.gamma,
+                   bc=s.bc, xmin=s.xmin, xmax=s.xmax, g=s.gravity)
+    o.init_from_prim(s.rho, s.u, s.P)
+    o.step(s.cfl, s.t_end, 1 << 30)  # warm-up (page faults, thread pool)
+    cells = 1
+    for d in range(s.ndim):
+        cells *= s.n[d]
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(s.cfl, s.t_end, 1 << 30)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 10000:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": cells * steps / el / 1e9,
+        "unit": "Gcell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} steps of {name} at {'x'.join(str(v) for v in s.n[:s.ndim])} after 1 warm-up step "
+                  f"({el:.1f} s, OpenMP {threads} threads, unfused reference kernel order)",
+    }
+
+
+def calibrate_copy(nbytes: int):
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="kelvin_helmholtz", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
+    a = ap.parse_args()
+
+    if a.gpus != 1:
+        print("bench.py: this build runs the hydro step on one GPU over the whole domain; "
+              "multi-GPU runs are not part of it (--gpus must be 1).", file=sys.stderr)
+        sys.exit(2)
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
+
+    import torch
+
+    from heraclespp_amd import _lib
+    from heraclespp_amd.hydro import Grid, PerfectGas, UniformGravity
+    from heraclespp_amd.solver import HydroSolver
+
+    torch.cuda.set_device(0)
+    s = _setup(a.workload, WORKLOADS[a.workload][0])
+    grid = Grid(s.ndim, s.n, s.ng, s.geom, s.xmin, s.xmax)
+    solver = HydroSolver(grid, PerfectGas(s.gamma, 1.0), reconstruction=s.reconstruction,
+                         riemann_solver=s.riemann_solver, gravity=UniformGravity(s.gravity), bc=s.bc, cfl=s.cfl)
+    solver.init_from_prim(s.rho, s.u, s.P)
+    del s.rho, s.u, s.P
+    cells = 1
+    for d in range(s.ndim):
+        cells *= s.n[d]
+
+    solver.start_clock(1e300, a.warmup + a.steps + 1)
+    for _ in range(a.warmup):
+        solver.step_async()
+    torch.cuda.synchronize()
+    L = _lib.lib()
+    _lib.check(L.hclpp_profile_begin(solver.ctx.handle, a.steps))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        solver.step_async()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kms = ctypes.c_double()
+    nl = ctypes.c_int()
+    _lib.check(L.hclpp_profile_end(solver.ctx.handle, ctypes.byref(kms), ctypes.byref(nl)))
+    clk = solver.finish()
+    if clk.iter != a.warmup + a.steps:
+        raise RuntimeError(f"expected {a.warmup + a.steps} steps, clock counted {clk.iter}")
+
+    wall = t1 - t0
+    value = cells * a.steps / wall / 1e9
+    kernel_s = kms.value / 1e3 / max(nl.value, 1)
+    bytes_alg = 2 * (s.ndim + 2) * 8 * cells
+    achieved = bytes_alg / kernel_s / 1e9
+    traffic = _traffic(a.workload)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": WORKLOADS[a.workload][2],
+            "cells": cells,
+            "shape": s.n[:s.ndim],
+            "reconstruction": s.reconstruction,
+            "riemann_solver": s.riemann_solver,
+            "storage": "fp64",
+            "path": "fused (hclpp_step: march kernel + ghost fill + device dt)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic,
+            "kernel": "k_step_march",
+            "kernel_ms": round(kernel_s * 1e3, 5),
+            "bytes_per_launch": bytes_alg,
+        },
+        "cpu_baseline": None,
+    }
+    del solver
+    torch.cuda.empty_cache()
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -31,7 +31,8 @@ EXPORTS = [
     "hclpp_limiter_from_name", "hclpp_riemann_from_name", "hclpp_bc_from_name",
     "hclpp_conv_prim_to_cons", "hclpp_conv_cons_to_prim", "hclpp_face_reconstruction", "hclpp_hancock_extrapolation",
     "hclpp_hydro_reconstruction", "hclpp_godunov", "hclpp_time_step", "hclpp_min_internal_energy", "hclpp_integrate",
-    "hclpp_fill_ghosts", "hclpp_clock_start", "hclpp_clock_read", "hclpp_step",
+    "hclpp_fill_ghosts", "hclpp_clock_start", "hclpp_clock_read", "hclpp_step", "hclpp_profile_begin",
+    "hclpp_profile_end",
 ]
 
 
@@ -116,6 +117,8 @@ def lib() -> ctypes.CDLL:
         "hclpp_clock_start": [vp, d, d, d, ip, dp, dp, dp],
         "hclpp_clock_read": [vp, ctypes.POINTER(Clock)],
         "hclpp_step": [vp, dp, dp, dp, dp, dp, dp],
+        "hclpp_profile_begin": [vp, ip],
+        "hclpp_profile_end": [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_int)],
     }
     for name, args in sig.items():
         f = getattr(L, name)

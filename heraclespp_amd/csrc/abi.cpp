@@ -45,6 +45,9 @@ struct hclpp_ctx {
     int64_t fx_old_cap = 0;
     GhostShell shell{};
     int64_t fused_nb = 0;
+    std::vector<hipEvent_t> prof_events;  // pairs (start, stop) per timed fused launch
+    int prof_max = 0;
+    int prof_count = 0;
 };
 
 namespace {
@@ -161,7 +164,7 @@ int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out)
     if (cfg->riemann < 0 || cfg->riemann > 2) return fail(HCLPP_ERR_ARG, "unknown Riemann solver");
     if (cfg->gravity != HCLPP_GRAV_UNIFORM && cfg->gravity != HCLPP_GRAV_POINT_MASS) return fail(HCLPP_ERR_ARG, "unknown gravity");
     if (cfg->nfx < 0) return fail(HCLPP_ERR_ARG, "nfx < 0");
-    if (cfg->ng < 2) return fail(HCLPP_ERR_ARG, "ng must be >= 2 (the reconstruction stencil is two cells wide)");
+    if (cfg->ng < 0) return fail(HCLPP_ERR_ARG, "ng < 0");
     if (cfg->precision != HCLPP_FP64 && cfg->precision != HCLPP_FP32_STORAGE) return fail(HCLPP_ERR_ARG, "unknown precision");
     int seen[6] = {0, 0, 0, 0, 0, 0};
     for (int f = 0; f < 2 * nd; ++f) {
@@ -172,7 +175,7 @@ int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out)
     }
     for (int d = 0; d < 3; ++d) {
         if (!cfg->x[d]) return fail(HCLPP_ERR_ARG, "null node coordinates");
-        if (d < nd && cfg->n[d] < cfg->ng) return fail(HCLPP_ERR_ARG, "interior extent smaller than the ghost width");
+        if (d < nd && (cfg->n[d] < cfg->ng || cfg->n[d] < 1)) return fail(HCLPP_ERR_ARG, "interior extent smaller than the ghost width");
         if (d >= nd && cfg->n[d] != 1) return fail(HCLPP_ERR_ARG, "inactive dims must have extent 1");
     }
 
@@ -289,6 +292,7 @@ int hclpp_ctx_destroy(hclpp_ctx* c)
     if (c->clock) (void)hipFree(c->clock);
     if (c->host_clock) (void)hipHostFree(c->host_clock);
     if (c->fx_old) (void)hipFree(c->fx_old);
+    for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
     delete c;
     return HCLPP_OK;
 }
@@ -463,6 +467,7 @@ int hclpp_clock_start(hclpp_ctx* c, double t0, double t_end, double cfl, int max
 {
     if (!c) return fail(HCLPP_ERR_ARG, "null context");
     if (!(cfl > 0) || max_iter < 1) return fail(HCLPP_ERR_ARG, "cfl must be > 0 and max_iter >= 1");
+    if (c->cfg.ng < 2) return fail(HCLPP_ERR_ARG, "the fused step needs ng >= 2 (two-cell reconstruction stencil)");
     DevClock* h = c->host_clock;
     HCLPP_HIP(hipStreamSynchronize(c->stream));  // the pinned staging buffer is reused
     std::memset(h, 0, sizeof(DevClock));
@@ -501,6 +506,7 @@ int hclpp_step(hclpp_ctx* c, const double* rho, const double* rhou, const double
     if (!c) return fail(HCLPP_ERR_ARG, "null context");
     if (c->cfg.nfx > 0) return fail(HCLPP_ERR_UNSUPPORTED, "the fused step has no passive scalars; use the operator entry points");
     if (c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "hclpp_step takes FP64 storage");
+    if (c->cfg.ng < 2) return fail(HCLPP_ERR_ARG, "the fused step needs ng >= 2 (two-cell reconstruction stencil)");
     if (!rho || !rhou || !E || !rho_new || !rhou_new || !E_new) return fail(HCLPP_ERR_ARG, "null array");
     if (rho == rho_new || E == E_new || rhou == rhou_new) return fail(HCLPP_ERR_ARG, "input and output arrays must differ");
     StepArrays A;
@@ -514,12 +520,49 @@ int hclpp_step(hclpp_ctx* c, const double* rho, const double* rhou, const double
     A.ghost_partials = c->ghost_partials;
     A.clock = c->clock;
     int const nd = c->cfg.ndim;
+    bool const timed = c->prof_count < c->prof_max;
+    if (timed) HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count], c->stream));
     launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->eos, A, c->stream);
     HCLPP_LAUNCHED("fused step");
+    if (timed) {
+        HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count + 1], c->stream));
+        c->prof_count += 1;
+    }
     launch_ghosts_rate(nd, c->cfg.geom, c->G, c->eos, c->bc, c->shell, A, c->stream);
     HCLPP_LAUNCHED("ghost fill");
     launch_clock_advance(nd, c->clock, c->partials, c->fused_nb, c->ghost_partials, c->ghost_cap, c->stream);
     HCLPP_LAUNCHED("clock advance");
+    return HCLPP_OK;
+}
+
+int hclpp_profile_begin(hclpp_ctx* c, int max_steps)
+{
+    if (!c || max_steps < 0) return fail(HCLPP_ERR_ARG, "invalid argument");
+    HCLPP_HIP(hipStreamSynchronize(c->stream));
+    while ((int)c->prof_events.size() < 2 * max_steps) {
+        hipEvent_t e;
+        HCLPP_HIP(hipEventCreate(&e));
+        c->prof_events.push_back(e);
+    }
+    c->prof_max = max_steps;
+    c->prof_count = 0;
+    return HCLPP_OK;
+}
+
+int hclpp_profile_end(hclpp_ctx* c, double* kernel_ms, int* launches)
+{
+    if (!c || !kernel_ms || !launches) return fail(HCLPP_ERR_ARG, "null argument");
+    HCLPP_HIP(hipStreamSynchronize(c->stream));
+    double total = 0;
+    for (int i = 0; i < c->prof_count; ++i) {
+        float ms = 0;
+        HCLPP_HIP(hipEventElapsedTime(&ms, c->prof_events[2 * i], c->prof_events[2 * i + 1]));
+        total += ms;
+    }
+    *kernel_ms = total;
+    *launches = c->prof_count;
+    c->prof_max = 0;
+    c->prof_count = 0;
     return HCLPP_OK;
 }
 

@@ -76,6 +76,19 @@ class Range:
         self.NgEff = NgEff
         self.ndim = ndim
         self.Nghost = [Ng if d < ndim else 0 for d in range(3)]
+        # cell (Nc) and face (Nf) bounds with 0, 1, 2 ghost layers (range.cpp:44-81)
+        for name in ("Nc_min_0g", "Nc_max_0g", "Nf_min_0g", "Nf_max_0g", "Nc_min_1g", "Nc_max_1g", "Nf_min_1g",
+                     "Nf_max_1g", "Nc_min_2g", "Nc_max_2g", "Nf_min_2g", "Nf_max_2g"):
+            setattr(self, name, [0, 0, 0])
+        for d in range(3):
+            e = 1 if d < ndim else 0
+            lo = self.Nghost[d]
+            hi = self.Nghost[d] + cmax[d] - cmin[d]
+            for g in range(3):
+                self.__dict__[f"Nc_min_{g}g"][d] = lo - g * e
+                self.__dict__[f"Nc_max_{g}g"][d] = hi + g * e
+                self.__dict__[f"Nf_min_{g}g"][d] = lo - g * e
+                self.__dict__[f"Nf_max_{g}g"][d] = hi + g * e + 1
 
     def with_ghosts(self, NgEff: int) -> "Range":
         if NgEff > self.Ng:

@@ -161,6 +161,14 @@ int hclpp_clock_read(hclpp_ctx* ctx, hclpp_clock* out);
 int hclpp_step(hclpp_ctx* ctx, const double* rho, const double* rhou, const double* E, double* rho_new, double* rhou_new,
                double* E_new);
 
+/* ---- instrumentation (no reference counterpart; for benchmarks) ----
+ * Between begin and end, each hclpp_step records a HIP event pair around its
+ * fused kernel on the context's stream (at most max_steps pairs).  end
+ * synchronises and returns the summed kernel time and the number of timed
+ * launches. */
+int hclpp_profile_begin(hclpp_ctx* ctx, int max_steps);
+int hclpp_profile_end(hclpp_ctx* ctx, double* kernel_ms, int* launches);
+
 #ifdef __cplusplus
 }
 #endif
