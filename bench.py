@@ -54,9 +54,9 @@ def _setup(name, kw):
     return setups.CONFIGS[name](**kw)
 
 
-def _traffic(workload: str):
+def _traffic(workload: str, precision: str):
     """Measured HBM bytes per fused-kernel launch from the committed rocprofv3 summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"k_step_march_{workload}_traffic.txt")
+    path = os.path.join(ROOT, "profiles", f"k_step_march_{workload}_{precision}_traffic.txt")
     if not os.path.exists(path):
         return None
     m = re.search(r"^traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", open(path).read(), re.M)
@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="kelvin_helmholtz", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
+                    help="fp32: FP32 storage and arithmetic with FP64 pressure recovery, dt and totals")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
@@ -136,7 +138,8 @@ def main():
     s = _setup(a.workload, WORKLOADS[a.workload][0])
     grid = Grid(s.ndim, s.n, s.ng, s.geom, s.xmin, s.xmax)
     solver = HydroSolver(grid, PerfectGas(s.gamma, 1.0), reconstruction=s.reconstruction,
-                         riemann_solver=s.riemann_solver, gravity=UniformGravity(s.gravity), bc=s.bc, cfl=s.cfl)
+                         riemann_solver=s.riemann_solver, gravity=UniformGravity(s.gravity), bc=s.bc, cfl=s.cfl,
+                         precision=a.precision)
     solver.init_from_prim(s.rho, s.u, s.P)
     del s.rho, s.u, s.P
     cells = 1
@@ -165,9 +168,10 @@ def main():
     wall = t1 - t0
     value = cells * a.steps / wall / 1e9
     kernel_s = kms.value / 1e3 / max(nl.value, 1)
-    bytes_alg = 2 * (s.ndim + 2) * 8 * cells
+    word = 8 if a.precision == "fp64" else 4
+    bytes_alg = 2 * (s.ndim + 2) * word * cells
     achieved = bytes_alg / kernel_s / 1e9
-    traffic = _traffic(a.workload)
+    traffic = _traffic(a.workload, a.precision)
     line = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -179,7 +183,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if a.precision == "fp64" else "f32 (FP64 pressure recovery, dt, totals)",
         "data": "synthetic",
         "config": {
             "workload": WORKLOADS[a.workload][2],
@@ -187,7 +191,7 @@ def main():
             "shape": s.n[:s.ndim],
             "reconstruction": s.reconstruction,
             "riemann_solver": s.riemann_solver,
-            "storage": "fp64",
+            "storage": a.precision,
             "path": "fused (hclpp_step: march kernel + ghost fill + device dt)",
         },
         "roofline": {

@@ -29,6 +29,7 @@ struct hclpp_ctx {
     hclpp_config cfg;
     GridDev G;
     PerfectGas eos;
+    PerfectGasMixed eos32;
     BcSpec bc;
     hipStream_t stream = nullptr;
     int device = 0;
@@ -194,6 +195,10 @@ int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out)
     c->eos.gamma = cfg->gamma;
     c->eos.gamma_m1 = cfg->gamma - 1;  // perfect_gas.cpp:13
     c->eos.mmw = cfg->mmw;
+    c->eos32.gamma = (float)cfg->gamma;
+    c->eos32.inv_gamma_m1 = (float)(1.0 / (cfg->gamma - 1));
+    c->eos32.gamma_m1 = cfg->gamma - 1;
+    c->eos32.gamma_d = cfg->gamma;
     for (int f = 0; f < 6; ++f) c->bc.kind[f] = f < 2 * nd ? cfg->bc[f] : 0;
 
     // host tables (mesh/grid.cpp:31-44, :134-155; geometry.cpp trigonometry; gravity.cpp:44-55)
@@ -463,9 +468,94 @@ int hclpp_fill_ghosts(hclpp_ctx* c, double* rho, double* rhou, double* E, double
     return HCLPP_OK;
 }
 
+static int clock_reset(hclpp_ctx* c, double t0, double t_end, double cfl, int max_iter)
+{
+    if (!(cfl > 0) || max_iter < 1) return fail(HCLPP_ERR_ARG, "cfl must be > 0 and max_iter >= 1");
+    if (c->cfg.ng < 2) return fail(HCLPP_ERR_ARG, "the fused step needs ng >= 2 (two-cell reconstruction stencil)");
+    DevClock* h = c->host_clock;
+    HCLPP_HIP(hipStreamSynchronize(c->stream));  // the pinned staging buffer is reused
+    std::memset(h, 0, sizeof(DevClock));
+    h->pub.t = t0;
+    h->pub.t_end = t_end;
+    h->pub.cfl = cfl;
+    h->pub.max_iter = max_iter;
+    h->pub.min_eint = NAN;
+    HCLPP_HIP(hipMemcpyAsync(c->clock, h, sizeof(DevClock), hipMemcpyHostToDevice, c->stream));
+    Range3 all;
+    for (int d = 0; d < 3; ++d) {
+        all.b[d] = 0;
+        all.e[d] = c->G.nwg[d];
+    }
+    return ensure_partials(c, reduce_blocks(all));
+}
+
+int hclpp_clock_start_f32(hclpp_ctx* c, double t0, double t_end, double cfl, int max_iter, const float* rho, const float* rhou, const float* E)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (c->cfg.precision != HCLPP_FP32_STORAGE) return fail(HCLPP_ERR_ARG, "the context was not created for FP32 storage");
+    int rc = clock_reset(c, t0, t_end, cfl, max_iter);
+    if (rc) return rc;
+    launch_state_rate_f32(c->cfg.ndim, c->cfg.geom, c->G, c->eos32, rho, rhou, E, c->partials, c->scalars, c->stream);
+    launch_clock_start(c->clock, c->scalars, c->stream);
+    HCLPP_LAUNCHED("clock start (fp32)");
+    HCLPP_HIP(hipStreamSynchronize(c->stream));
+    return HCLPP_OK;
+}
+
+int hclpp_step_f32(hclpp_ctx* c, const float* rho, const float* rhou, const float* E, float* rho_new, float* rhou_new, float* E_new)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (c->cfg.nfx > 0) return fail(HCLPP_ERR_UNSUPPORTED, "the fused step has no passive scalars; use the operator entry points");
+    if (c->cfg.precision != HCLPP_FP32_STORAGE) return fail(HCLPP_ERR_ARG, "the context was not created for FP32 storage");
+    if (c->cfg.ng < 2) return fail(HCLPP_ERR_ARG, "the fused step needs ng >= 2 (two-cell reconstruction stencil)");
+    if (!rho || !rhou || !E || !rho_new || !rhou_new || !E_new) return fail(HCLPP_ERR_ARG, "null array");
+    if (rho == rho_new || E == E_new || rhou == rhou_new) return fail(HCLPP_ERR_ARG, "input and output arrays must differ");
+    StepArraysF A;
+    A.rho = rho;
+    A.rhou = rhou;
+    A.E = E;
+    A.rho_new = rho_new;
+    A.rhou_new = rhou_new;
+    A.E_new = E_new;
+    A.partials = c->partials;
+    A.ghost_partials = c->ghost_partials;
+    A.clock = c->clock;
+    int const nd = c->cfg.ndim;
+    bool const timed = c->prof_count < c->prof_max;
+    if (timed) HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count], c->stream));
+    launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->eos32, A, c->stream);
+    HCLPP_LAUNCHED("fused step (fp32)");
+    if (timed) {
+        HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count + 1], c->stream));
+        c->prof_count += 1;
+    }
+    launch_ghosts_rate(nd, c->cfg.geom, c->G, c->eos32, c->bc, c->shell, A, c->stream);
+    HCLPP_LAUNCHED("ghost fill (fp32)");
+    launch_clock_advance(nd, c->clock, c->partials, c->fused_nb, c->ghost_partials, c->ghost_cap, c->stream);
+    HCLPP_LAUNCHED("clock advance");
+    return HCLPP_OK;
+}
+
+int hclpp_cast_f64_to_f32(hclpp_ctx* c, const double* in, float* out, long long n)
+{
+    if (!c || (n > 0 && (!in || !out))) return fail(HCLPP_ERR_ARG, "null argument");
+    launch_cast(in, out, (int64_t)n, c->stream);
+    HCLPP_LAUNCHED("cast f64->f32");
+    return HCLPP_OK;
+}
+
+int hclpp_cast_f32_to_f64(hclpp_ctx* c, const float* in, double* out, long long n)
+{
+    if (!c || (n > 0 && (!in || !out))) return fail(HCLPP_ERR_ARG, "null argument");
+    launch_cast(in, out, (int64_t)n, c->stream);
+    HCLPP_LAUNCHED("cast f32->f64");
+    return HCLPP_OK;
+}
+
 int hclpp_clock_start(hclpp_ctx* c, double t0, double t_end, double cfl, int max_iter, const double* rho, const double* rhou, const double* E)
 {
     if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_ARG, "the context was created for FP32 storage");
     if (!(cfl > 0) || max_iter < 1) return fail(HCLPP_ERR_ARG, "cfl must be > 0 and max_iter >= 1");
     if (c->cfg.ng < 2) return fail(HCLPP_ERR_ARG, "the fused step needs ng >= 2 (two-cell reconstruction stencil)");
     DevClock* h = c->host_clock;

@@ -21,17 +21,21 @@ struct DevClock {
 // energy, [2..2+NV) sums of the conserved variables times dV.
 constexpr int kPartialStride = 8;
 
-struct StepArrays {
-    const double* __restrict__ rho;
-    const double* __restrict__ rhou;
-    const double* __restrict__ E;
-    double* __restrict__ rho_new;
-    double* __restrict__ rhou_new;
-    double* __restrict__ E_new;
+// Arrays of one fused step; S is the storage type (double, or float for FP32 storage).
+template <class S>
+struct StepArraysT {
+    const S* __restrict__ rho;
+    const S* __restrict__ rhou;
+    const S* __restrict__ E;
+    S* __restrict__ rho_new;
+    S* __restrict__ rhou_new;
+    S* __restrict__ E_new;
     double* __restrict__ partials;
     double* __restrict__ ghost_partials;
     DevClock* clock;
 };
+using StepArrays = StepArraysT<double>;
+using StepArraysF = StepArraysT<float>;
 
 struct FusedLayout {
     int ntile[2];  // in-plane tile counts
@@ -65,10 +69,18 @@ struct MarchShape<3> {  // 32 x 8 columns (30 x 6 outputs + halo) marching along
 
 int64_t fused_blocks(const GridDev& G, int nd, FusedLayout* layout);
 void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, hipStream_t s);
+void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGasMixed& eos, const StepArraysF& A, hipStream_t s);
 GhostShell make_ghost_shell(const GridDev& G, int nd);
 int64_t ghost_blocks(const GhostShell& B);
 void launch_ghosts_rate(int nd, int geom, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const GhostShell& B, const StepArrays& A,
                         hipStream_t s);
+void launch_ghosts_rate(int nd, int geom, const GridDev& G, const PerfectGasMixed& eos, const BcSpec& bc, const GhostShell& B,
+                        const StepArraysF& A, hipStream_t s);
+// max CFL rate of a whole FP32-storage conservative state (FP64 arithmetic) into out[0]
+void launch_state_rate_f32(int nd, int geom, const GridDev& G, const PerfectGasMixed& eos, const float* rho, const float* rhou, const float* E,
+                           double* partials, double* out, hipStream_t s);
+void launch_cast(const double* in, float* out, int64_t n, hipStream_t s);
+void launch_cast(const float* in, double* out, int64_t n, hipStream_t s);
 void launch_clock_start(DevClock* c, const double* inv, hipStream_t s);
 void launch_clock_advance(int nd, DevClock* c, const double* partials, int64_t nblocks, const double* ghost_partials, int64_t nghost,
                           hipStream_t s);

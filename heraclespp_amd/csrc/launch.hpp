@@ -68,6 +68,8 @@ void launch_time_step_rate(int nd, int geom, const GridDev& G, const Range3& r, 
 void launch_min_internal_energy(int nd, const GridDev& G, const Range3& r, const double* rho, const double* rhou, const double* E,
                                 double* partials, double* out, hipStream_t s);
 void launch_integrate(const GridDev& G, int nd, int geom, const Range3& r, const double* var, double* partials, double* out, hipStream_t s);
+// Fold n partials (max) into out[0] in a fixed order.
+void launch_fold_max(const double* partials, int64_t n, double* out, hipStream_t s);
 // Ghost fill on conservative arrays (boundary_distribute.hpp:42-76 semantics, single rank)
 void launch_fill_ghosts(int nd, const GridDev& G, const BcSpec& bc, double* rho, double* rhou, double* E, double* fx, int nfx, hipStream_t s);
 

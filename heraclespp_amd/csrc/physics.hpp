@@ -2,9 +2,19 @@
 //
 // A state is a small register array q[NV] with NV = NDIM + 2: q[0] density,
 // q[1..NDIM] momentum (or velocity), q[NDIM+1] total energy (or pressure).
-// Every expression keeps the operation order of the reference so the FP64
-// path is bit-identical to it when compiled with -ffp-contract=off; the
-// comments cite the reference file:line each function follows.
+//
+// Every routine is generic over an equation-of-state policy that fixes the
+// arithmetic:
+//  * PerfectGas (FP64): the reference arithmetic -- every expression keeps
+//    the reference operation order and rounds like it (built with
+//    -ffp-contract=off), so results are bit-identical to the reference;
+//  * PerfectGasMixed (FP32): state and fluxes in FP32 with hardware reciprocal
+//    divisions, while the pressure recovery P = (gamma-1)(E - |m|^2 / 2 rho),
+//    which cancels at high Mach number, is evaluated in FP64.
+// Division is always written through Eos::div so the FP64 policy keeps the
+// correctly-rounded quotient and the FP32 policy uses v_rcp_f32.  Halving is
+// a multiplication by 0.5, which is exact in IEEE arithmetic.
+// Comments cite the reference file:line each function follows.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,37 +33,45 @@ enum Limiter { kConstant = 0, kVanLeer = 1, kMinmod = 2, kVanAlbada = 3 };
 enum Solver { kHLL = 0, kHLLC = 1, kLowMach = 2 };
 enum Geom { kCartesian = 0, kSpherical = 1 };
 
-// thermodynamics/perfect_gas.hpp:17-90 (gamma, gamma-1, mean molecular weight)
+template <int ND, class T = double>
+struct State {
+    static constexpr int NV = ND + 2;
+    T q[NV];
+    __host__ __device__ __forceinline__ T& operator[](int i) { return q[i]; }
+    __host__ __device__ __forceinline__ T operator[](int i) const { return q[i]; }
+};
+
+// euler_equations.hpp:41-49: kinetic energy of a conservative state (policy-free helper)
+template <int ND, class T, class Div>
+__host__ __device__ __forceinline__ T kinetic_cons_with(const State<ND, T>& c, Div div)
+{
+    T n = 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) n += c[1 + d] * c[1 + d];
+    return div(n, c[0]) * T(0.5);
+}
+
+// thermodynamics/perfect_gas.hpp:17-90 (gamma, gamma-1, mean molecular weight): FP64 reference arithmetic
 struct PerfectGas {
+    using real = double;
     double gamma;
     double gamma_m1;
     double mmw;
 
+    __host__ __device__ static __forceinline__ double div(double a, double b) { return a / b; }
     // perfect_gas.hpp:69-72
-    __host__ __device__ __forceinline__ double temp_from_pres(double rho, double P) const
-    {
-        return P * mmw * kMp / (rho * kKb);
-    }
+    __host__ __device__ __forceinline__ double temp_from_pres(double rho, double P) const { return P * mmw * kMp / (rho * kKb); }
     // perfect_gas.hpp:52-56
-    __host__ __device__ __forceinline__ double evol_from_temp(double rho, double T) const
-    {
-        return rho * kKb * T / (mmw * kMp * gamma_m1);
-    }
+    __host__ __device__ __forceinline__ double evol_from_temp(double rho, double T) const { return rho * kKb * T / (mmw * kMp * gamma_m1); }
     // perfect_gas.hpp:46-50
-    __host__ __device__ __forceinline__ double evol_from_pres(double rho, double P) const
-    {
-        return evol_from_temp(rho, temp_from_pres(rho, P));
-    }
+    __host__ __device__ __forceinline__ double evol_from_pres(double rho, double P) const { return evol_from_temp(rho, temp_from_pres(rho, P)); }
     // perfect_gas.hpp:74-78
     __host__ __device__ __forceinline__ double temp_from_evol(double rho, double evol) const
     {
         return evol * mmw * kMp * gamma_m1 / (rho * kKb);
     }
     // perfect_gas.hpp:64-67
-    __host__ __device__ __forceinline__ double pres_from_temp(double rho, double T) const
-    {
-        return rho * kKb * T / (mmw * kMp);
-    }
+    __host__ __device__ __forceinline__ double pres_from_temp(double rho, double T) const { return rho * kKb * T / (mmw * kMp); }
     // perfect_gas.hpp:58-62
     __host__ __device__ __forceinline__ double pres_from_evol(double rho, double evol) const
     {
@@ -61,61 +79,100 @@ struct PerfectGas {
     }
     // perfect_gas.hpp:80-83
     __host__ __device__ __forceinline__ double sound(double rho, double P) const { return sqrt(gamma * P / rho); }
+    // euler_equations.hpp:61-65, :111-122: pressure of a conservative state
+    template <int ND>
+    __host__ __device__ __forceinline__ double pressure_of_cons(const State<ND, double>& c) const
+    {
+        double const evol = c[ND + 1] - kinetic_cons_with(c, div);
+        return pres_from_evol(c[0], evol);
+    }
+    template <int ND>
+    __host__ __device__ __forceinline__ double internal_energy_fp64(const State<ND, double>& c) const
+    {
+        return c[ND + 1] - kinetic_cons_with(c, div);
+    }
 };
 
-template <int ND>
-struct State {
-    static constexpr int NV = ND + 2;
-    double q[NV];
-    __host__ __device__ __forceinline__ double& operator[](int i) { return q[i]; }
-    __host__ __device__ __forceinline__ double operator[](int i) const { return q[i]; }
-};
+// FP32 state and arithmetic; FP64 pressure recovery.  Not a parity path: the
+// accuracy bar is an L1 distance to the FP64 result.
+struct PerfectGasMixed {
+    using real = float;
+    float gamma;
+    float inv_gamma_m1;
+    double gamma_m1;
+    double gamma_d;
 
-// euler_equations.hpp:41-49: kinetic energy of a conservative state
-template <int ND>
-__host__ __device__ __forceinline__ double kinetic_cons(const State<ND>& c)
-{
-    double n = 0;
+    __host__ __device__ static __forceinline__ float div(float a, float b)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return a * __builtin_amdgcn_rcpf(b);
+#else
+        return a / b;
+#endif
+    }
+    __host__ __device__ __forceinline__ float evol_from_pres(float, float P) const { return P * inv_gamma_m1; }
+    __host__ __device__ __forceinline__ float sound(float rho, float P) const { return sqrtf(gamma * div(P, rho)); }
+    // E - |m|^2 / (2 rho) and (gamma - 1) times it, in FP64
+    template <int ND>
+    __host__ __device__ __forceinline__ double internal_energy_fp64(const State<ND, float>& c) const
+    {
+        double n = 0;
 #pragma unroll
-    for (int d = 0; d < ND; ++d) n += c[1 + d] * c[1 + d];
-    return n / c[0] / 2;
+        for (int d = 0; d < ND; ++d) n += (double)c[1 + d] * (double)c[1 + d];
+        return (double)c[ND + 1] - n / (double)c[0] * 0.5;
+    }
+    template <int ND>
+    __host__ __device__ __forceinline__ float pressure_of_cons(const State<ND, float>& c) const
+    {
+        return (float)(gamma_m1 * internal_energy_fp64(c));
+    }
+};
+
+// euler_equations.hpp:41-49
+template <int ND, class T>
+__host__ __device__ __forceinline__ T kinetic_cons(const State<ND, T>& c)
+{
+    if constexpr (sizeof(T) == 8) {
+        return kinetic_cons_with(c, PerfectGas::div);
+    } else {
+        return kinetic_cons_with(c, PerfectGasMixed::div);
+    }
 }
 
 // euler_equations.hpp:51-59: kinetic energy of a primitive state
-template <int ND>
-__host__ __device__ __forceinline__ double kinetic_prim(const State<ND>& p)
+template <int ND, class T>
+__host__ __device__ __forceinline__ T kinetic_prim(const State<ND, T>& p)
 {
-    double n = 0;
+    T n = 0;
 #pragma unroll
     for (int d = 0; d < ND; ++d) n += p[1 + d] * p[1 + d];
-    return p[0] * n / 2;
+    return p[0] * n * T(0.5);
 }
 
-// euler_equations.hpp:61-65
+// euler_equations.hpp:61-65 (FP64 reference arithmetic)
 template <int ND>
-__host__ __device__ __forceinline__ double internal_energy(const State<ND>& c)
+__host__ __device__ __forceinline__ double internal_energy(const State<ND, double>& c)
 {
     return c[ND + 1] - kinetic_cons(c);
 }
 
 // euler_equations.hpp:111-122
-template <int ND>
-__host__ __device__ __forceinline__ State<ND> cons_to_prim(const State<ND>& c, const PerfectGas& eos)
+template <int ND, class Eos>
+__host__ __device__ __forceinline__ State<ND, typename Eos::real> cons_to_prim(const State<ND, typename Eos::real>& c, const Eos& eos)
 {
-    State<ND> p;
-    double const evol = internal_energy(c);
+    State<ND, typename Eos::real> p;
     p[0] = c[0];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) p[1 + d] = c[1 + d] / c[0];
-    p[ND + 1] = eos.pres_from_evol(c[0], evol);
+    for (int d = 0; d < ND; ++d) p[1 + d] = Eos::div(c[1 + d], c[0]);
+    p[ND + 1] = eos.pressure_of_cons(c);
     return p;
 }
 
 // euler_equations.hpp:124-134
-template <int ND>
-__host__ __device__ __forceinline__ State<ND> prim_to_cons(const State<ND>& p, const PerfectGas& eos)
+template <int ND, class Eos>
+__host__ __device__ __forceinline__ State<ND, typename Eos::real> prim_to_cons(const State<ND, typename Eos::real>& p, const Eos& eos)
 {
-    State<ND> c;
+    State<ND, typename Eos::real> c;
     c[0] = p[0];
 #pragma unroll
     for (int d = 0; d < ND; ++d) c[1 + d] = p[0] * p[1 + d];
@@ -124,12 +181,13 @@ __host__ __device__ __forceinline__ State<ND> prim_to_cons(const State<ND>& p, c
 }
 
 // euler_equations.hpp:72-86: physical flux along `dir` from a primitive state
-template <int ND>
-__host__ __device__ __forceinline__ State<ND> flux_of_prim(const State<ND>& p, int dir, const PerfectGas& eos)
+template <int ND, class Eos>
+__host__ __device__ __forceinline__ State<ND, typename Eos::real> flux_of_prim(const State<ND, typename Eos::real>& p, int dir, const Eos& eos)
 {
-    State<ND> f;
-    double const E = kinetic_prim(p) + eos.evol_from_pres(p[0], p[ND + 1]);
-    double const un = p[1 + dir];
+    using T = typename Eos::real;
+    State<ND, T> f;
+    T const E = kinetic_prim(p) + eos.evol_from_pres(p[0], p[ND + 1]);
+    T const un = p[1 + dir];
     f[0] = p[0] * un;
 #pragma unroll
     for (int d = 0; d < ND; ++d) f[1 + d] = p[0] * un * p[1 + d];
@@ -139,13 +197,13 @@ __host__ __device__ __forceinline__ State<ND> flux_of_prim(const State<ND>& p, i
 }
 
 // euler_equations.hpp:93-109: physical flux along `dir` from a conservative state
-template <int ND>
-__host__ __device__ __forceinline__ State<ND> flux_of_cons(const State<ND>& c, int dir, const PerfectGas& eos)
+template <int ND, class Eos>
+__host__ __device__ __forceinline__ State<ND, typename Eos::real> flux_of_cons(const State<ND, typename Eos::real>& c, int dir, const Eos& eos)
 {
-    State<ND> f;
-    double const evol = internal_energy(c);
-    double const P = eos.pres_from_evol(c[0], evol);
-    double const u = c[1 + dir] / c[0];
+    using T = typename Eos::real;
+    State<ND, T> f;
+    T const P = eos.pressure_of_cons(c);
+    T const u = Eos::div(c[1 + dir], c[0]);
     f[0] = u * c[0];
 #pragma unroll
     for (int d = 0; d < ND; ++d) f[1 + d] = u * c[1 + d];
@@ -156,123 +214,125 @@ __host__ __device__ __forceinline__ State<ND> flux_of_cons(const State<ND>& c, i
 
 // ---------------------------------------------------------------- limiters
 // slope_limiters.hpp:18-87.  diffR = (v[i+1]-v[i])/dx, diffL = (v[i]-v[i-1])/dx
-__host__ __device__ __forceinline__ double limited_slope(int lim, double dR, double dL)
-{
-    if (lim == kConstant) return 0;
-    if (dL * dR > 0) {
-        double const r = dR / dL;
-        if (lim == kVanLeer) return (1. / 2) * (dR + dL) * (4 * r) / ((r + 1) * (r + 1));
-        if (lim == kMinmod) {
-            double const mm = 2 * fmin(1., r) / (1 + r);
-            return mm * (dL + dR) / 2;
-        }
-        return (1. / 2) * (dR + dL) * (2 * r) / ((r * r) + 1);
-    }
-    return 0;
-}
-
-template <int LIM>
-__host__ __device__ __forceinline__ double limited_slope_t(double dR, double dL)
+template <int LIM, class Eos, class T>
+__host__ __device__ __forceinline__ T limited_slope_t(T dR, T dL)
 {
     if constexpr (LIM == kConstant) {
         return 0;
     } else {
         if (dL * dR > 0) {
-            double const r = dR / dL;
+            T const r = Eos::div(dR, dL);
             if constexpr (LIM == kVanLeer) {
-                return (1. / 2) * (dR + dL) * (4 * r) / ((r + 1) * (r + 1));
+                return Eos::div(T(0.5) * (dR + dL) * (4 * r), (r + 1) * (r + 1));
             } else if constexpr (LIM == kMinmod) {
-                double const mm = 2 * fmin(1., r) / (1 + r);
-                return mm * (dL + dR) / 2;
+                T const mm = Eos::div(2 * fmin(T(1), r), 1 + r);
+                return mm * (dL + dR) * T(0.5);
             } else {
-                return (1. / 2) * (dR + dL) * (2 * r) / ((r * r) + 1);
+                return Eos::div(T(0.5) * (dR + dL) * (2 * r), (r * r) + 1);
             }
         }
         return 0;
     }
 }
 
+template <int LIM>
+__host__ __device__ __forceinline__ double limited_slope_t(double dR, double dL)
+{
+    return limited_slope_t<LIM, PerfectGas, double>(dR, dL);
+}
+
 // ---------------------------------------------------------------- Riemann solvers
 // riemann_solver.hpp:60-64
-__host__ __device__ __forceinline__ double hll_component(double UL, double UR, double FL, double FR, double wsL, double wsR)
+template <class Eos, class T>
+__host__ __device__ __forceinline__ T hll_component(T UL, T UR, T FL, T FR, T wsL, T wsR)
 {
-    return ((wsR * FL) - (wsL * FR) + (wsL * wsR * (UR - UL))) / (wsR - wsL);
+    return Eos::div((wsR * FL) - (wsL * FR) + (wsL * wsR * (UR - UL)), wsR - wsL);
 }
 
 // riemann_solver.hpp:30-58
-template <int ND>
-__host__ __device__ __forceinline__ State<ND> riemann_hll(const State<ND>& cL, const State<ND>& cR, int dir, const PerfectGas& eos)
+template <int ND, class Eos>
+__host__ __device__ __forceinline__ State<ND, typename Eos::real> riemann_hll(const State<ND, typename Eos::real>& cL,
+                                                                             const State<ND, typename Eos::real>& cR, int dir, const Eos& eos)
 {
-    State<ND> const pL = cons_to_prim(cL, eos);
-    State<ND> const pR = cons_to_prim(cR, eos);
-    double const sL = eos.sound(pL[0], pL[ND + 1]);
-    double const sR = eos.sound(pR[0], pR[ND + 1]);
-    double const wsL = fmin(pL[1 + dir] - sL, pR[1 + dir] - sR);
-    double const wsR = fmax(pL[1 + dir] + sL, pR[1 + dir] + sR);
-    double const nL = fmin(wsL, 0.);
-    double const pR_ = fmax(wsR, 0.);
-    State<ND> const fL = flux_of_prim(pL, dir, eos);
-    State<ND> const fR = flux_of_prim(pR, dir, eos);
-    State<ND> f;
+    using T = typename Eos::real;
+    State<ND, T> const pL = cons_to_prim(cL, eos);
+    State<ND, T> const pR = cons_to_prim(cR, eos);
+    T const sL = eos.sound(pL[0], pL[ND + 1]);
+    T const sR = eos.sound(pR[0], pR[ND + 1]);
+    T const wsL = fmin(pL[1 + dir] - sL, pR[1 + dir] - sR);
+    T const wsR = fmax(pL[1 + dir] + sL, pR[1 + dir] + sR);
+    T const nL = fmin(wsL, T(0));
+    T const pR_ = fmax(wsR, T(0));
+    State<ND, T> const fL = flux_of_prim(pL, dir, eos);
+    State<ND, T> const fR = flux_of_prim(pR, dir, eos);
+    State<ND, T> f;
 #pragma unroll
-    for (int v = 0; v < ND + 2; ++v) f[v] = hll_component(cL[v], cR[v], fL[v], fR[v], nL, pR_);
+    for (int v = 0; v < ND + 2; ++v) f[v] = hll_component<Eos>(cL[v], cR[v], fL[v], fR[v], nL, pR_);
     return f;
 }
 
 // riemann_solver.hpp:136-140
-__host__ __device__ __forceinline__ double hllc_component(double F, double U, double Us, double ws) { return F + (ws * (Us - U)); }
+template <class T>
+__host__ __device__ __forceinline__ T hllc_component(T F, T U, T Us, T ws)
+{
+    return F + (ws * (Us - U));
+}
 
 // riemann_solver.hpp:70-134
-template <int ND>
-__host__ __device__ __forceinline__ State<ND> riemann_hllc(const State<ND>& cL, const State<ND>& cR, int dir, const PerfectGas& eos)
+template <int ND, class Eos>
+__host__ __device__ __forceinline__ State<ND, typename Eos::real> riemann_hllc(const State<ND, typename Eos::real>& cL,
+                                                                              const State<ND, typename Eos::real>& cR, int dir, const Eos& eos)
 {
-    State<ND> const pL = cons_to_prim(cL, eos);
-    State<ND> const pR = cons_to_prim(cR, eos);
-    double const sL = eos.sound(pL[0], pL[ND + 1]);
-    double const sR = eos.sound(pR[0], pR[ND + 1]);
-    double const uL = pL[1 + dir];
-    double const uR = pR[1 + dir];
-    double const wsL = fmin(uL - sL, uR - sR);
-    double const wsR = fmax(uL + sL, uR + sR);
-    State<ND> const fL = flux_of_prim(pL, dir, eos);
-    State<ND> const fR = flux_of_prim(pR, dir, eos);
-    double const rcL = pL[0] * (wsL - uL);
-    double const rcR = pR[0] * (wsR - uR);
-    double const ustar = (pR[ND + 1] - pL[ND + 1] + (rcL * uL) - (rcR * uR)) / (rcL - rcR);
-    double const pstar = 1. / 2 * (pL[ND + 1] + pR[ND + 1] + (rcL * (ustar - uL)) + (rcR * (ustar - uR)));
+    using T = typename Eos::real;
+    State<ND, T> const pL = cons_to_prim(cL, eos);
+    State<ND, T> const pR = cons_to_prim(cR, eos);
+    T const sL = eos.sound(pL[0], pL[ND + 1]);
+    T const sR = eos.sound(pR[0], pR[ND + 1]);
+    T const uL = pL[1 + dir];
+    T const uR = pR[1 + dir];
+    T const wsL = fmin(uL - sL, uR - sR);
+    T const wsR = fmax(uL + sL, uR + sR);
+    State<ND, T> const fL = flux_of_prim(pL, dir, eos);
+    State<ND, T> const fR = flux_of_prim(pR, dir, eos);
+    T const rcL = pL[0] * (wsL - uL);
+    T const rcR = pR[0] * (wsR - uR);
+    T const ustar = Eos::div(pR[ND + 1] - pL[ND + 1] + (rcL * uL) - (rcR * uR), rcL - rcR);
+    T const pstar = T(0.5) * (pL[ND + 1] + pR[ND + 1] + (rcL * (ustar - uL)) + (rcR * (ustar - uR)));
     bool const left = ustar > 0;
-    double const S = left ? wsL : wsR;
-    State<ND> const& cs = left ? cL : cR;
-    State<ND> const& ps = left ? pL : pR;
-    State<ND> const& fs = left ? fL : fR;
-    double const un = ps[1 + dir];
+    T const S = left ? wsL : wsR;
+    State<ND, T> const& cs = left ? cL : cR;
+    State<ND, T> const& ps = left ? pL : pR;
+    State<ND, T> const& fs = left ? fL : fR;
+    T const un = ps[1 + dir];
     if (wsL * wsR > 0) return fs;
-    State<ND> f;
-    double const rho_star = cs[0] * (S - un) / (S - ustar);
+    State<ND, T> f;
+    T const rho_star = Eos::div(cs[0] * (S - un), S - ustar);
     f[0] = hllc_component(fs[0], cs[0], rho_star, S);
 #pragma unroll
     for (int d = 0; d < ND; ++d) f[1 + d] = rho_star * ustar * ps[1 + d];
     f[1 + dir] = (rho_star * ustar * ustar) + pstar;
-    double const E_star = ((S - un) / (S - ustar) * cs[ND + 1]) + ((ustar - un) / (S - ustar) * ((ustar * cs[0] * (S - un)) + ps[ND + 1]));
+    T const E_star = (Eos::div(S - un, S - ustar) * cs[ND + 1]) + (Eos::div(ustar - un, S - ustar) * ((ustar * cs[0] * (S - un)) + ps[ND + 1]));
     f[ND + 1] = hllc_component(fs[ND + 1], cs[ND + 1], E_star, S);
     return f;
 }
 
 // riemann_solver.hpp:146-180 ("Low Mach" splitting solver)
-template <int ND>
-__host__ __device__ __forceinline__ State<ND> riemann_lowmach(const State<ND>& cL, const State<ND>& cR, int dir, const PerfectGas& eos)
+template <int ND, class Eos>
+__host__ __device__ __forceinline__ State<ND, typename Eos::real> riemann_lowmach(const State<ND, typename Eos::real>& cL,
+                                                                                 const State<ND, typename Eos::real>& cR, int dir, const Eos& eos)
 {
-    State<ND> const pL = cons_to_prim(cL, eos);
-    State<ND> const pR = cons_to_prim(cR, eos);
-    double const sL = eos.sound(pL[0], pL[ND + 1]);
-    double const sR = eos.sound(pR[0], pR[ND + 1]);
-    double const a = 1.1 * fmax(pL[0] * sL, pR[0] * sR);
-    double const ustar = ((pL[1 + dir] + pR[1 + dir]) / 2) - (1 / (2 * a) * (pR[ND + 1] - pL[ND + 1]));
-    double const Ma = fabs(ustar) / fmin(sL, sR);
-    double const theta = fmin(1., Ma);
-    double const Pstar = ((pL[ND + 1] + pR[ND + 1]) / 2) - ((theta * a) / 2 * (pR[1 + dir] - pL[1 + dir]));
-    State<ND> const& cs = (ustar > 0) ? cL : cR;
-    State<ND> f;
+    using T = typename Eos::real;
+    State<ND, T> const pL = cons_to_prim(cL, eos);
+    State<ND, T> const pR = cons_to_prim(cR, eos);
+    T const sL = eos.sound(pL[0], pL[ND + 1]);
+    T const sR = eos.sound(pR[0], pR[ND + 1]);
+    T const a = T(1.1) * fmax(pL[0] * sL, pR[0] * sR);
+    T const ustar = ((pL[1 + dir] + pR[1 + dir]) * T(0.5)) - (Eos::div(T(1), 2 * a) * (pR[ND + 1] - pL[ND + 1]));
+    T const Ma = Eos::div(fabs(ustar), fmin(sL, sR));
+    T const theta = fmin(T(1), Ma);
+    T const Pstar = ((pL[ND + 1] + pR[ND + 1]) * T(0.5)) - ((theta * a) * T(0.5) * (pR[1 + dir] - pL[1 + dir]));
+    State<ND, T> const& cs = (ustar > 0) ? cL : cR;
+    State<ND, T> f;
     f[0] = ustar * cs[0];
 #pragma unroll
     for (int d = 0; d < ND; ++d) f[1 + d] = ustar * cs[1 + d];
@@ -281,8 +341,9 @@ __host__ __device__ __forceinline__ State<ND> riemann_lowmach(const State<ND>& c
     return f;
 }
 
-template <int RS, int ND>
-__host__ __device__ __forceinline__ State<ND> riemann(const State<ND>& cL, const State<ND>& cR, int dir, const PerfectGas& eos)
+template <int RS, int ND, class Eos>
+__host__ __device__ __forceinline__ State<ND, typename Eos::real> riemann(const State<ND, typename Eos::real>& cL,
+                                                                         const State<ND, typename Eos::real>& cR, int dir, const Eos& eos)
 {
     if constexpr (RS == kHLL) {
         return riemann_hll(cL, cR, dir, eos);
@@ -296,28 +357,30 @@ __host__ __device__ __forceinline__ State<ND> riemann(const State<ND>& cL, const
 // ---------------------------------------------------------------- spherical sources
 // hydro/source_terms.hpp:18-77.  Trigonometry comes from host tables:
 // cot_mid = cos(m)/sin(m) with m = (x1 + x1p)/2; sn = sin(x1), snp = sin(x1p).
-__host__ __device__ __forceinline__ double src_pressure(double dtodv, double PL, double PR, double dS, double dSp)
+template <class Eos = PerfectGas, class T>
+__host__ __device__ __forceinline__ T src_pressure(T dtodv, T PL, T PR, T dS, T dSp)
 {
-    return dtodv * (PL + PR) / 2 * (dSp - dS);
+    return dtodv * (PL + PR) * T(0.5) * (dSp - dS);
 }
-__host__ __device__ __forceinline__ double src_centrifugal(double dtodv, double rL, double rR, double uL, double uR, double dS, double dSp)
+template <class Eos = PerfectGas, class T>
+__host__ __device__ __forceinline__ T src_centrifugal(T dtodv, T rL, T rR, T uL, T uR, T dS, T dSp)
 {
-    return dtodv * ((rL * uL * uL) + (rR * uR * uR)) / 2 * (dSp - dS) / 2;
+    return dtodv * ((rL * uL * uL) + (rR * uR * uR)) * T(0.5) * (dSp - dS) * T(0.5);
 }
-__host__ __device__ __forceinline__ double src_coriolis_r(double dtodv, double x0, double x0p, double rL, double rR, double uLr, double uRr,
-                                                          double uLo, double uRo, double dS, double dSp)
+template <class Eos = PerfectGas, class T>
+__host__ __device__ __forceinline__ T src_coriolis_r(T dtodv, T x0, T x0p, T rL, T rR, T uLr, T uRr, T uLo, T uRo, T dS, T dSp)
 {
-    return dtodv * (x0p - x0) / (x0p + x0) * ((rR * uRo * uRr * dSp) + (rL * uLo * uLr * dS));
+    return Eos::div(dtodv * (x0p - x0), x0p + x0) * ((rR * uRo * uRr * dSp) + (rL * uLo * uLr * dS));
 }
-__host__ __device__ __forceinline__ double src_centrifugal_th(double dtodv, double cot_mid, double rL, double rR, double uLp, double uRp, double dS,
-                                                              double dSp)
+template <class Eos = PerfectGas, class T>
+__host__ __device__ __forceinline__ T src_centrifugal_th(T dtodv, T cot_mid, T rL, T rR, T uLp, T uRp, T dS, T dSp)
 {
-    return dtodv * ((rL * uLp * uLp) + (rR * uRp * uRp)) / 2 * cot_mid * (dSp - dS) / 2;
+    return dtodv * ((rL * uLp * uLp) + (rR * uRp * uRp)) * T(0.5) * cot_mid * (dSp - dS) * T(0.5);
 }
-__host__ __device__ __forceinline__ double src_coriolis_th(double dtodv, double sn, double snp, double rL, double rR, double uLt, double uRt,
-                                                           double uLp, double uRp, double dS, double dSp)
+template <class Eos = PerfectGas, class T>
+__host__ __device__ __forceinline__ T src_coriolis_th(T dtodv, T sn, T snp, T rL, T rR, T uLt, T uRt, T uLp, T uRp, T dS, T dSp)
 {
-    return dtodv * (snp - sn) / (snp + sn) * ((rR * uRp * uRt * dSp) + (rL * uLp * uLt * dS));
+    return Eos::div(dtodv * (snp - sn), snp + sn) * ((rR * uRp * uRt * dSp) + (rL * uLp * uLt * dS));
 }
 
 }  // namespace hclpp

@@ -570,6 +570,11 @@ void launch_time_step_cons(int nd, int geom, const GridDev& G, const Range3& r, 
     hipLaunchKernelGGL(k_fold<kMax>, dim3(1), dim3(1024), 0, s, partials, reduce_blocks(r), out);
 }
 
+void launch_fold_max(const double* partials, int64_t n, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fold<kMax>, dim3(1), dim3(1024), 0, s, partials, n, out);
+}
+
 void launch_min_internal_energy(int nd, const GridDev& G, const Range3& r, const double* rho, const double* rhou, const double* E,
                                 double* partials, double* out, hipStream_t s)
 {

@@ -1,6 +1,6 @@
 // step_fused.hip -- one MUSCL-Hancock + Riemann + Godunov step in a single pass over HBM.
 //
-// Design (gfx950, FP64):
+// Design (gfx950):
 //  * The grid is cut into in-plane tiles (x in 2D; x-y in 3D) and segments
 //    along the last active dim (the "march" dim).  A workgroup owns one tile
 //    plus a one-cell halo ring ("Hancock region", one thread per column) and
@@ -9,22 +9,28 @@
 //    to primitives, publishes plane h's primitives (plus a two-cell ring) in
 //    LDS, reconstructs all faces (limited slopes along every dim: in-plane
 //    neighbours from LDS, march neighbours from registers), converts the face
-//    states to conservative form and applies the Hancock half step -- exactly
-//    the reference's face_reconstruction / conv_prim_to_cons / Hancock
-//    kernels, for one cell, in registers.
+//    states to conservative form and applies the Hancock half step -- the
+//    reference's face_reconstruction / conv_prim_to_cons / Hancock kernels for
+//    one cell, in registers.
 //  * In-plane faces: each column publishes its right face state in LDS, the
 //    right neighbour solves the Riemann problem once per face, fluxes are
 //    exchanged back through LDS.  March-dim faces: the right face state of
 //    plane h-1 is kept in registers and paired with plane h's left state.
 //  * The Godunov update of a cell accumulates dims in the reference order
-//    (flux, geometric sources, gravity per dim, march dim last), so every
-//    conserved value is bit-identical to the reference's unfused sequence.
+//    (flux, geometric sources, gravity per dim, march dim last), so with the
+//    FP64 policy every conserved value is bit-identical to the reference's
+//    unfused sequence.
 //  * Face states never touch HBM: per cell-update the kernel reads the
 //    conservative state once (plus halo re-reads served mostly by L2) and
-//    writes it once.  The epilogue reduces the CFL rate of the new state,
-//    the minimum internal energy of the consumed state and the conserved
-//    totals; per-workgroup partials are folded in a fixed order.
+//    writes it once.  The epilogue reduces (in FP64) the CFL rate of the new
+//    state, the minimum internal energy of the consumed state and the
+//    conserved totals; per-workgroup partials are folded in a fixed order.
+//  * The kernels are generic over the precision policy (physics.hpp):
+//    PerfectGas streams FP64; PerfectGasMixed streams FP32 (half the bytes)
+//    with FP64 pressure recovery and FP64 reductions.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "fused.hpp"
 #include "launch.hpp"
@@ -49,10 +55,10 @@ __device__ __forceinline__ void ijk_of(int a, int b, int m, int& i, int& j, int&
     else { i = a; j = b; k = m; }
 }
 
-template <int ND>
-__device__ __forceinline__ State<ND> load_cons(const StepArrays& A, const GridDev& G, int64_t id)
+template <int ND, class S>
+__device__ __forceinline__ State<ND, S> load_cons(const StepArraysT<S>& A, const GridDev& G, int64_t id)
 {
-    State<ND> c;
+    State<ND, S> c;
     c[0] = A.rho[id];
 #pragma unroll
     for (int d = 0; d < ND; ++d) c[1 + d] = A.rhou[id + G.plane * d];
@@ -60,8 +66,8 @@ __device__ __forceinline__ State<ND> load_cons(const StepArrays& A, const GridDe
     return c;
 }
 
-template <int ND>
-__device__ __forceinline__ void store_cons(const StepArrays& A, const GridDev& G, int64_t id, const State<ND>& c)
+template <int ND, class S>
+__device__ __forceinline__ void store_cons(const StepArraysT<S>& A, const GridDev& G, int64_t id, const State<ND, S>& c)
 {
     A.rho_new[id] = c[0];
 #pragma unroll
@@ -69,68 +75,101 @@ __device__ __forceinline__ void store_cons(const StepArrays& A, const GridDev& G
     A.E_new[id] = c[ND + 1];
 }
 
-// limited_linear_reconstruction.hpp:53-62 for one variable along one dim
-template <int LIM>
-__device__ __forceinline__ void reconstruct(double vm, double v, double vp, double dlm, double dl, double dlp, double& lo, double& hi)
+// CFL rate of a conservative state at (i,j,k) (time_step.hpp:61-76), in FP64.  With the
+// FP64 policy this is the reference's arithmetic (conv_cons_to_prim, then the rate).
+template <int ND, int GEOM, class Eos>
+__device__ __forceinline__ double state_rate(const GridDev& G, const Eos& eos, int i, int j, int k, const State<ND, typename Eos::real>& c)
 {
-    double const slope = limited_slope_t<LIM>((vp - v) / ((dl + dlp) / 2), (v - vm) / ((dlm + dl) / 2));
-    lo = v - ((dl / 2) * slope);
-    hi = v + ((dl / 2) * slope);
+    if constexpr (std::is_same_v<Eos, PerfectGas>) {
+        State<ND> const p = cons_to_prim(c, eos);
+        return cfl_rate<ND, GEOM>(G, i, j, k, &p.q[1], eos.sound(p[0], p[ND + 1]));
+    } else {
+        double const rho = c[0];
+        double u[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) u[d] = (double)c[1 + d] / rho;
+        double const P = eos.gamma_m1 * eos.internal_energy_fp64(c);
+        return cfl_rate<ND, GEOM>(G, i, j, k, u, sqrt(eos.gamma_d * P / rho));
+    }
+}
+
+// Internal energy E - |m|^2 / 2 rho of a conservative state in FP64 (internal_energy.cpp:25-40).
+template <int ND, class Eos>
+__device__ __forceinline__ double state_eint(const Eos& eos, const State<ND, typename Eos::real>& c)
+{
+    if constexpr (std::is_same_v<Eos, PerfectGas>) return internal_energy(c);
+    else return eos.internal_energy_fp64(c);
+}
+
+// limited_linear_reconstruction.hpp:53-62 for one variable along one dim
+template <int LIM, class Eos, class T>
+__device__ __forceinline__ void reconstruct(T vm, T v, T vp, T dlm, T dl, T dlp, T& lo, T& hi)
+{
+    T const slope = limited_slope_t<LIM, Eos, T>(Eos::div(vp - v, (dl + dlp) * T(0.5)), Eos::div(v - vm, (dlm + dl) * T(0.5)));
+    lo = v - ((dl * T(0.5)) * slope);
+    hi = v + ((dl * T(0.5)) * slope);
 }
 
 // Spherical sources of one dim for the cell's own face states (godunov_scheme.hpp:165-256),
 // applied to the accumulator in the reference order.
-template <int ND, int GEOM>
-__device__ __forceinline__ void godunov_sources(const GridDev& G, const PerfectGas& eos, int dim, int i, int j, double dtodv, double dS,
-                                                double dSp, const State<ND>& vL, const State<ND>& vR, State<ND>& un)
+template <class Eos, int ND, int GEOM>
+__device__ __forceinline__ void godunov_sources(const GridDev& G, const Eos& eos, int dim, int i, int j, typename Eos::real dtodv,
+                                                typename Eos::real dS, typename Eos::real dSp, const State<ND, typename Eos::real>& vL,
+                                                const State<ND, typename Eos::real>& vR, State<ND, typename Eos::real>& un)
 {
+    using T = typename Eos::real;
     if constexpr (GEOM == kSpherical) {
         if (dim > 1) return;
-        State<ND> const pL = cons_to_prim(vL, eos);
-        State<ND> const pR = cons_to_prim(vR, eos);
+        State<ND, T> const pL = cons_to_prim(vL, eos);
+        State<ND, T> const pR = cons_to_prim(vR, eos);
+        T const x0 = T(G.x[0][i]), x0p = T(G.x[0][i + 1]);
         if (dim == 0) {
-            un[1] += src_pressure(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+            un[1] += src_pressure<Eos>(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
             if constexpr (ND >= 2) {
-                un[1] += src_centrifugal(dtodv, pL[0], pR[0], pL[2], pR[2], dS, dSp);
-                un[2] -= src_coriolis_r(dtodv, G.x[0][i], G.x[0][i + 1], pL[0], pR[0], pL[1], pR[1], pL[2], pR[2], dS, dSp);
+                un[1] += src_centrifugal<Eos>(dtodv, pL[0], pR[0], pL[2], pR[2], dS, dSp);
+                un[2] -= src_coriolis_r<Eos>(dtodv, x0, x0p, pL[0], pR[0], pL[1], pR[1], pL[2], pR[2], dS, dSp);
             }
             if constexpr (ND == 3) {
-                un[1] += src_centrifugal(dtodv, pL[0], pR[0], pL[3], pR[3], dS, dSp);
-                un[3] -= src_coriolis_r(dtodv, G.x[0][i], G.x[0][i + 1], pL[0], pR[0], pL[1], pR[1], pL[3], pR[3], dS, dSp);
+                un[1] += src_centrifugal<Eos>(dtodv, pL[0], pR[0], pL[3], pR[3], dS, dSp);
+                un[3] -= src_coriolis_r<Eos>(dtodv, x0, x0p, pL[0], pR[0], pL[1], pR[1], pL[3], pR[3], dS, dSp);
             }
         } else {
-            un[2] += src_pressure(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+            un[2] += src_pressure<Eos>(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
             if constexpr (ND == 3) {
-                un[2] += src_centrifugal_th(dtodv, G.cot_mid[j], pL[0], pR[0], pL[3], pR[3], dS, dSp);
-                un[3] -= src_coriolis_th(dtodv, G.sin_node[j], G.sin_node[j + 1], pL[0], pR[0], pL[2], pR[2], pL[3], pR[3], dS, dSp);
+                un[2] += src_centrifugal_th<Eos>(dtodv, T(G.cot_mid[j]), pL[0], pR[0], pL[3], pR[3], dS, dSp);
+                un[3] -= src_coriolis_th<Eos>(dtodv, T(G.sin_node[j]), T(G.sin_node[j + 1]), pL[0], pR[0], pL[2], pR[2], pL[3], pR[3], dS,
+                                              dSp);
             }
         }
     }
 }
 
 // MUSCL-Hancock predictor of one cell (hydro_reconstruction.hpp:99-128 and
-// extrapolation_reconstruction.hpp:90-312 with nfx = 0): prim[dim][side] are
-// the neighbours (side 0 = minus, 1 = plus) along each dim.
-template <int ND, int GEOM, int LIM>
-__device__ __forceinline__ void predict_faces(const GridDev& G, const PerfectGas& eos, double dth, int i, int j, int k,
-                                              const State<ND>& p, const State<ND> (&nb)[ND][2], State<ND> (&fs)[2][ND])
+// extrapolation_reconstruction.hpp:90-312 with nfx = 0): nb[dim][side] are
+// the primitive neighbours (side 0 = minus, 1 = plus) along each dim.
+template <class Eos, int ND, int GEOM, int LIM>
+__device__ __forceinline__ void predict_faces(const GridDev& G, const Eos& eos, typename Eos::real dth, double dth_d, int i, int j, int k,
+                                              const State<ND, typename Eos::real>& p, const State<ND, typename Eos::real> (&nb)[ND][2],
+                                              State<ND, typename Eos::real> (&fs)[2][ND])
 {
+    using T = typename Eos::real;
     constexpr int NV = ND + 2;
-    State<ND> fp[2][ND];
+    State<ND, T> fp[2][ND];
     int const idx[3] = {i, j, k};
 #pragma unroll
     for (int dim = 0; dim < ND; ++dim) {
-        double const dl = G.dx[dim][idx[dim]];
-        double const dlm = G.dx[dim][idx[dim] - 1];
-        double const dlp = G.dx[dim][idx[dim] + 1];
+        T const dl = T(G.dx[dim][idx[dim]]);
+        T const dlm = T(G.dx[dim][idx[dim] - 1]);
+        T const dlp = T(G.dx[dim][idx[dim] + 1]);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) reconstruct<LIM>(nb[dim][0][v], p[v], nb[dim][1][v], dlm, dl, dlp, fp[0][dim][v], fp[1][dim][v]);
+        for (int v = 0; v < NV; ++v)
+            reconstruct<LIM, Eos>(nb[dim][0][v], p[v], nb[dim][1][v], dlm, dl, dlp, fp[0][dim][v], fp[1][dim][v]);
     }
 #pragma unroll
     for (int dim = 0; dim < ND; ++dim)
 #pragma unroll
         for (int s = 0; s < 2; ++s) fs[s][dim] = prim_to_cons(fp[s][dim], eos);
-    double rho_old[2][ND], mom_old[2][ND][ND];
+    T rho_old[2][ND], mom_old[2][ND][ND];
 #pragma unroll
     for (int dim = 0; dim < ND; ++dim)
 #pragma unroll
@@ -139,15 +178,16 @@ __device__ __forceinline__ void predict_faces(const GridDev& G, const PerfectGas
 #pragma unroll
             for (int v = 0; v < ND; ++v) mom_old[s][dim][v] = fs[s][dim][1 + v];
         }
-    double const dtodv = dth / cell_volume<ND, GEOM>(G, i, j, k);
+    T const dtodv = T(dth_d / cell_volume<ND, GEOM>(G, i, j, k));
+    T const x0 = T(G.x[0][i]), x0p = T(G.x[0][i + 1]);
 #pragma unroll
     for (int dim = 0; dim < ND; ++dim) {
-        State<ND> const& pL = fp[0][dim];
-        State<ND> const& pR = fp[1][dim];
-        State<ND> const fL = flux_of_prim(pL, dim, eos);
-        State<ND> const fR = flux_of_prim(pR, dim, eos);
-        double const dS = face_area<ND, GEOM>(G, dim, i, j, k);
-        double const dSp = face_area<ND, GEOM>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2));
+        State<ND, T> const& pL = fp[0][dim];
+        State<ND, T> const& pR = fp[1][dim];
+        State<ND, T> const fL = flux_of_prim(pL, dim, eos);
+        State<ND, T> const fR = flux_of_prim(pR, dim, eos);
+        T const dS = T(face_area<ND, GEOM>(G, dim, i, j, k));
+        T const dSp = T(face_area<ND, GEOM>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2)));
 #pragma unroll
         for (int pos = 0; pos < ND; ++pos)
 #pragma unroll
@@ -160,27 +200,27 @@ __device__ __forceinline__ void predict_faces(const GridDev& G, const PerfectGas
 #pragma unroll
                 for (int s = 0; s < 2; ++s) {
                     if (dim == 0) {
-                        fs[s][pos][1] += src_pressure(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+                        fs[s][pos][1] += src_pressure<Eos>(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
                         if constexpr (ND >= 2) {
-                            fs[s][pos][1] += src_centrifugal(dtodv, pL[0], pR[0], pL[2], pR[2], dS, dSp);
-                            fs[s][pos][2] -= src_coriolis_r(dtodv, G.x[0][i], G.x[0][i + 1], pL[0], pR[0], pL[1], pR[1], pL[2], pR[2], dS, dSp);
+                            fs[s][pos][1] += src_centrifugal<Eos>(dtodv, pL[0], pR[0], pL[2], pR[2], dS, dSp);
+                            fs[s][pos][2] -= src_coriolis_r<Eos>(dtodv, x0, x0p, pL[0], pR[0], pL[1], pR[1], pL[2], pR[2], dS, dSp);
                         }
                         if constexpr (ND == 3) {
-                            fs[s][pos][1] += src_centrifugal(dtodv, pL[0], pR[0], pL[3], pR[3], dS, dSp);
-                            fs[s][pos][3] -= src_coriolis_r(dtodv, G.x[0][i], G.x[0][i + 1], pL[0], pR[0], pL[1], pR[1], pL[3], pR[3], dS, dSp);
+                            fs[s][pos][1] += src_centrifugal<Eos>(dtodv, pL[0], pR[0], pL[3], pR[3], dS, dSp);
+                            fs[s][pos][3] -= src_coriolis_r<Eos>(dtodv, x0, x0p, pL[0], pR[0], pL[1], pR[1], pL[3], pR[3], dS, dSp);
                         }
                     }
                     if (dim == 1) {
-                        fs[s][pos][2] += src_pressure(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+                        fs[s][pos][2] += src_pressure<Eos>(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
                         if constexpr (ND == 3) {
-                            fs[s][pos][2] += src_centrifugal_th(dtodv, G.cot_mid[j], pL[0], pR[0], pL[3], pR[3], dS, dSp);
-                            fs[s][pos][3] -= src_coriolis_th(dtodv, G.sin_node[j], G.sin_node[j + 1], pL[0], pR[0], pL[2], pR[2], pL[3],
-                                                             pR[3], dS, dSp);
+                            fs[s][pos][2] += src_centrifugal_th<Eos>(dtodv, T(G.cot_mid[j]), pL[0], pR[0], pL[3], pR[3], dS, dSp);
+                            fs[s][pos][3] -= src_coriolis_th<Eos>(dtodv, T(G.sin_node[j]), T(G.sin_node[j + 1]), pL[0], pR[0], pL[2], pR[2],
+                                                                  pL[3], pR[3], dS, dSp);
                         }
                     }
                 }
         }
-        double const g = gravity(G, i, dim);
+        T const g = T(gravity(G, i, dim));
 #pragma unroll
         for (int pos = 0; pos < ND; ++pos)
 #pragma unroll
@@ -192,20 +232,21 @@ __device__ __forceinline__ void predict_faces(const GridDev& G, const PerfectGas
 }
 
 // Safe stand-in for cells outside the array (never feeds a valid output).
-template <int ND>
-__device__ __forceinline__ State<ND> dummy_prim()
+template <int ND, class T>
+__device__ __forceinline__ State<ND, T> dummy_state()
 {
-    State<ND> p;
-    p[0] = 1.;
+    State<ND, T> p;
+    p[0] = 1;
 #pragma unroll
-    for (int d = 0; d < ND; ++d) p[1 + d] = 0.;
-    p[ND + 1] = 1.;
+    for (int d = 0; d < ND; ++d) p[1 + d] = 0;
+    p[ND + 1] = 1;
     return p;
 }
 
-template <int ND, int GEOM, int RS, int LIM, int HX, int HY, int NT>
-__global__ __launch_bounds__(NT) void k_step_march(GridDev G, PerfectGas eos, StepArrays A, FusedLayout L)
+template <class Eos, int ND, int GEOM, int RS, int LIM, int HX, int HY, int NT>
+__global__ __launch_bounds__(NT) void k_step_march(GridDev G, Eos eos, StepArraysT<typename Eos::real> A, FusedLayout L)
 {
+    using T = typename Eos::real;
     constexpr int NV = ND + 2;
     constexpr int NP = ND - 1;               // in-plane dims
     constexpr int MD = ND - 1;               // march dim
@@ -214,14 +255,16 @@ __global__ __launch_bounds__(NT) void k_step_march(GridDev G, PerfectGas eos, St
     constexpr int TX = (ND >= 2) ? HX - 2 : 1;
     constexpr int TY = (ND == 3) ? HY - 2 : 1;
     constexpr int NXCH = NP > 0 ? NP : 1;
-    __shared__ double s_prim[NV][PY][PX];
-    __shared__ double s_xch[NXCH][NV][HY][HX];
+    __shared__ T s_prim[NV][PY][PX];
+    __shared__ T s_xch[NXCH][NV][HY][HX];
     __shared__ double s_red[NT / 64];
 
     DevClock const* clk = A.clock;
     if (clk->pub.done || clk->pub.status) return;
-    double const dt = clk->pub.dt;
-    double const dth = dt / 2;
+    double const dt_d = clk->pub.dt;
+    double const dth_d = dt_d / 2;
+    T const dt = T(dt_d);
+    T const dth = T(dth_d);
 
     int const tid = threadIdx.x;
     // XCD-aware block order: blocks that share an XCD (b % 8) get consecutive tiles.
@@ -280,18 +323,18 @@ __global__ __launch_bounds__(NT) void k_step_march(GridDev G, PerfectGas eos, St
 #pragma unroll
     for (int v = 0; v < NV; ++v) r_sum[v] = 0;
 
-    auto prim_at = [&](int ca, int cb, int m) -> State<ND> {
+    auto prim_at = [&](int ca, int cb, int m) -> State<ND, T> {
         bool ok = m >= 0 && m < G.nwg[MD];
         if constexpr (ND >= 2) ok = ok && ca >= 0 && ca < G.nwg[0];
         if constexpr (ND == 3) ok = ok && cb >= 0 && cb < G.nwg[1];
-        if (!ok) return dummy_prim<ND>();
+        if (!ok) return dummy_state<ND, T>();
         return cons_to_prim(load_cons<ND>(A, G, cell_at<ND>(G, ca, cb, m)), eos);
     };
 
     // pipeline registers
-    State<ND> pm, p0, pp, c0, cp;
-    State<ND> fsR_prev, Fm_prev, part_prev;
-    double old_rho_prev = 0, old_mom_prev = 0, src_prev = 0;
+    State<ND, T> pm, p0, pp, c0, cp;
+    State<ND, T> fsR_prev, Fm_prev, part_prev;
+    T old_rho_prev = 0, old_mom_prev = 0, src_prev = 0;
     int const zfirst = seg_ok ? zs - 1 : 0;
     int const zlast = seg_ok ? ze : -1;
     if (col_in_array) {
@@ -299,9 +342,9 @@ __global__ __launch_bounds__(NT) void k_step_march(GridDev G, PerfectGas eos, St
         c0 = load_cons<ND>(A, G, cell_at<ND>(G, a, bb, zfirst));
         p0 = cons_to_prim(c0, eos);
     } else {
-        pm = dummy_prim<ND>();
-        p0 = dummy_prim<ND>();
-        c0 = dummy_prim<ND>();
+        pm = dummy_state<ND, T>();
+        p0 = dummy_state<ND, T>();
+        c0 = dummy_state<ND, T>();
     }
 
     for (int h = zfirst; h <= zlast; ++h) {
@@ -310,8 +353,8 @@ __global__ __launch_bounds__(NT) void k_step_march(GridDev G, PerfectGas eos, St
             cp = load_cons<ND>(A, G, cell_at<ND>(G, a, bb, h + 1));
             pp = cons_to_prim(cp, eos);
         } else {
-            cp = dummy_prim<ND>();
-            pp = dummy_prim<ND>();
+            cp = dummy_state<ND, T>();
+            pp = dummy_state<ND, T>();
         }
         // ---- plane h primitives in LDS (own column + ring)
         if constexpr (ND >= 2) {
@@ -331,7 +374,7 @@ __global__ __launch_bounds__(NT) void k_step_march(GridDev G, PerfectGas eos, St
                     else if (r < 2 * (PX - 2) + (PY - 2)) { px = 0; py = 1 + r - 2 * (PX - 2); }
                     else { px = PX - 1; py = 1 + r - 2 * (PX - 2) - (PY - 2); }
                 }
-                State<ND> const pr = prim_at(ox - 2 + px, (ND == 3) ? oy - 2 + py : 0, h);
+                State<ND, T> const pr = prim_at(ox - 2 + px, (ND == 3) ? oy - 2 + py : 0, h);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) s_prim[v][py][px] = pr[v];
             }
@@ -342,9 +385,9 @@ __global__ __launch_bounds__(NT) void k_step_march(GridDev G, PerfectGas eos, St
         bool const out_plane = h >= zs && h < ze;
 
         // ---- predictor of plane h
-        State<ND> fs[2][ND];
+        State<ND, T> fs[2][ND];
         if (hancock_ok) {
-            State<ND> nb3[ND][2];
+            State<ND, T> nb3[ND][2];
             if constexpr (ND >= 2) {
                 int const py0 = (ND == 3) ? hy + 1 : 0;
 #pragma unroll
@@ -362,7 +405,7 @@ __global__ __launch_bounds__(NT) void k_step_march(GridDev G, PerfectGas eos, St
             }
             nb3[MD][0] = pm;
             nb3[MD][1] = pp;
-            predict_faces<ND, GEOM, LIM>(G, eos, dth, ci, cj, ck, p0, nb3, fs);
+            predict_faces<Eos, ND, GEOM, LIM>(G, eos, dth, dth_d, ci, cj, ck, p0, nb3, fs);
         } else {
 #pragma unroll
             for (int s = 0; s < 2; ++s)
@@ -371,17 +414,17 @@ __global__ __launch_bounds__(NT) void k_step_march(GridDev G, PerfectGas eos, St
         }
 
         // ---- in-plane fluxes and the in-plane part of the update of plane h
-        State<ND> part = c0;
-        double old_rho = c0[0], old_mom = c0[1 + MD], src_m = 0;
-        double dtodv = 0;
+        State<ND, T> part = c0;
+        T old_rho = c0[0], old_mom = c0[1 + MD], src_m = 0;
+        T dtodv = 0;
         if (out_plane) {
             if (out_col) {
-                dtodv = dt / cell_volume<ND, GEOM>(G, ci, cj, ck);
+                dtodv = T(dt_d / cell_volume<ND, GEOM>(G, ci, cj, ck));
                 // min internal energy of the consumed state (internal_energy.cpp:25-40)
-                r_emin = reduce_combine<kMin>(r_emin, internal_energy(c0));
+                r_emin = reduce_combine<kMin>(r_emin, state_eint<ND>(eos, c0));
             }
             if constexpr (ND >= 2) {
-                State<ND> FLp[NXCH], FRp[NXCH];
+                State<ND, T> FLp[NXCH], FRp[NXCH];
 #pragma unroll
                 for (int p = 0; p < NP; ++p) {
                     __syncthreads();
@@ -393,9 +436,9 @@ __global__ __launch_bounds__(NT) void k_step_march(GridDev G, PerfectGas eos, St
                     bool need;
                     if (p == 0) need = hancock_ok && hx >= 1 && (ND == 2 || (hy >= 1 && hy <= TY));
                     else need = hancock_ok && hy >= 1 && hx >= 1 && hx <= TX;
-                    State<ND> F;
+                    State<ND, T> F;
                     if (need) {
-                        State<ND> left;
+                        State<ND, T> left;
 #pragma unroll
                         for (int v = 0; v < NV; ++v) left[v] = (p == 0) ? s_xch[p][v][hy][hx - 1] : s_xch[p][v][hy - 1][hx];
                         F = riemann<RS>(left, fs[0][p], p, eos);
@@ -417,64 +460,52 @@ __global__ __launch_bounds__(NT) void k_step_march(GridDev G, PerfectGas eos, St
                 if (out_col) {
 #pragma unroll
                     for (int p = 0; p < NP; ++p) {
-                        double const dS = face_area<ND, GEOM>(G, p, ci, cj, ck);
-                        double const dSp = face_area<ND, GEOM>(G, p, ci + (p == 0), cj + (p == 1), ck);
+                        T const dS = T(face_area<ND, GEOM>(G, p, ci, cj, ck));
+                        T const dSp = T(face_area<ND, GEOM>(G, p, ci + (p == 0), cj + (p == 1), ck));
 #pragma unroll
                         for (int v = 0; v < NV; ++v) part[v] += dtodv * ((FLp[p][v] * dS) - (FRp[p][v] * dSp));
-                        godunov_sources<ND, GEOM>(G, eos, p, ci, cj, dtodv, dS, dSp, fs[0][p], fs[1][p], part);
-                        double const g = gravity(G, ci, p);
+                        godunov_sources<Eos, ND, GEOM>(G, eos, p, ci, cj, dtodv, dS, dSp, fs[0][p], fs[1][p], part);
+                        T const g = T(gravity(G, ci, p));
                         part[1 + p] += dt * g * c0[0];
                         part[ND + 1] += dt * g * c0[1 + p];
                     }
                 }
             }
-            // geometric pressure source of the march dim (2D spherical: theta)
-            if constexpr (GEOM == kSpherical && MD == 1 && ND == 2) {
+            // geometric pressure source of the march dim (1D spherical: r; 2D spherical: theta)
+            if constexpr (GEOM == kSpherical && MD <= 1) {
                 if (out_col) {
-                    State<ND> const pL = cons_to_prim(fs[0][MD], eos);
-                    State<ND> const pR = cons_to_prim(fs[1][MD], eos);
-                    double const dS = face_area<ND, GEOM>(G, MD, ci, cj, ck);
-                    double const dSp = face_area<ND, GEOM>(G, MD, ci, cj + 1, ck);
-                    src_m = src_pressure(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
-                }
-            }
-            if constexpr (GEOM == kSpherical && MD == 0) {
-                // 1D spherical: the march dim is r
-                if (out_col) {
-                    State<ND> const pL = cons_to_prim(fs[0][0], eos);
-                    State<ND> const pR = cons_to_prim(fs[1][0], eos);
-                    double const dS = face_area<ND, GEOM>(G, 0, ci, cj, ck);
-                    double const dSp = face_area<ND, GEOM>(G, 0, ci + 1, cj, ck);
-                    src_m = src_pressure(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+                    State<ND, T> const pL = cons_to_prim(fs[0][MD], eos);
+                    State<ND, T> const pR = cons_to_prim(fs[1][MD], eos);
+                    T const dS = T(face_area<ND, GEOM>(G, MD, ci, cj, ck));
+                    T const dSp = T(face_area<ND, GEOM>(G, MD, ci + (MD == 0), cj + (MD == 1), ck));
+                    src_m = src_pressure<Eos>(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
                 }
             }
         }
 
         // ---- march-dim face h-1/2 and completion of plane h-1
         if (h >= zs && out_col) {
-            State<ND> const Fm = riemann<RS>(fsR_prev, fs[0][MD], MD, eos);
+            State<ND, T> const Fm = riemann<RS>(fsR_prev, fs[0][MD], MD, eos);
             int const hm = h - 1;
             if (hm >= zs) {
                 int pi, pj, pk;
                 ijk_of<ND>(a, bb, hm, pi, pj, pk);
-                double const dtodv_m = dt / cell_volume<ND, GEOM>(G, pi, pj, pk);
-                double const dS = face_area<ND, GEOM>(G, MD, pi, pj, pk);
-                double const dSp = face_area<ND, GEOM>(G, MD, ci, cj, ck);
-                State<ND> un = part_prev;
+                double const vol = cell_volume<ND, GEOM>(G, pi, pj, pk);
+                T const dtodv_m = T(dt_d / vol);
+                T const dS = T(face_area<ND, GEOM>(G, MD, pi, pj, pk));
+                T const dSp = T(face_area<ND, GEOM>(G, MD, ci, cj, ck));
+                State<ND, T> un = part_prev;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) un[v] += dtodv_m * ((Fm_prev[v] * dS) - (Fm[v] * dSp));
                 if constexpr (GEOM == kSpherical && MD <= 1) un[1 + MD] += src_prev;
-                double const g = gravity(G, pi, MD);
+                T const g = T(gravity(G, pi, MD));
                 un[1 + MD] += dt * g * old_rho_prev;
                 un[ND + 1] += dt * g * old_mom_prev;
                 store_cons<ND>(A, G, cell_at<ND>(G, a, bb, hm), un);
-                // epilogue: CFL rate of the new state (time_step.hpp:61-76) and totals
-                State<ND> const pn = cons_to_prim(un, eos);
-                double const c = eos.sound(pn[0], pn[ND + 1]);
-                r_inv = reduce_combine<kMax>(r_inv, cfl_rate<ND, GEOM>(G, pi, pj, pk, &pn.q[1], c));
-                double const dv = cell_volume<ND, GEOM>(G, pi, pj, pk);
+                // epilogue: CFL rate of the new state and the conserved totals (FP64)
+                r_inv = reduce_combine<kMax>(r_inv, state_rate<ND, GEOM>(G, eos, pi, pj, pk, un));
 #pragma unroll
-                for (int v = 0; v < NV; ++v) r_sum[v] += un[v] * dv;
+                for (int v = 0; v < NV; ++v) r_sum[v] += (double)un[v] * vol;
             }
             Fm_prev = Fm;
         }
@@ -522,9 +553,10 @@ __device__ __forceinline__ int ghost_map(int kind, int idx, int ng, int n, int e
     return idx;
 }
 
-template <int ND, int GEOM>
-__global__ __launch_bounds__(256) void k_ghosts_rate(GridDev G, PerfectGas eos, BcSpec bc, GhostShell B, StepArrays A)
+template <class Eos, int ND, int GEOM>
+__global__ __launch_bounds__(256) void k_ghosts_rate(GridDev G, Eos eos, BcSpec bc, GhostShell B, StepArraysT<typename Eos::real> A)
 {
+    using T = typename Eos::real;
     __shared__ double s_red[4];
     DevClock const* clk = A.clock;
     if (clk->pub.done || clk->pub.status) return;
@@ -549,11 +581,11 @@ __global__ __launch_bounds__(256) void k_ghosts_rate(GridDev G, PerfectGas eos, 
         }
         int64_t const dst = flat(G, i, j, k);
         int64_t const s = flat(G, src[0], src[1], src[2]);
-        State<ND> c;
+        State<ND, T> c;
         c[0] = A.rho_new[s];
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
-            double const m = A.rhou_new[s + G.plane * d];
+            T const m = A.rhou_new[s + G.plane * d];
             c[1 + d] = flip[d] ? -m : m;
         }
         c[ND + 1] = A.E_new[s];
@@ -561,11 +593,39 @@ __global__ __launch_bounds__(256) void k_ghosts_rate(GridDev G, PerfectGas eos, 
 #pragma unroll
         for (int d = 0; d < ND; ++d) A.rhou_new[dst + G.plane * d] = c[1 + d];
         A.E_new[dst] = c[ND + 1];
-        State<ND> const p = cons_to_prim(c, eos);
-        rate = cfl_rate<ND, GEOM>(G, i, j, k, &p.q[1], eos.sound(p[0], p[ND + 1]));
+        rate = state_rate<ND, GEOM>(G, eos, i, j, k, c);
     }
     rate = block_reduce<kMax>(rate, s_red, threadIdx.x, 256);
     if (threadIdx.x == 0) A.ghost_partials[blockIdx.x] = rate;
+}
+
+// Max CFL rate over every cell (ghosts included) of an FP32-storage state.
+template <int ND, int GEOM>
+__global__ __launch_bounds__(256) void k_state_rate_f32(GridDev G, PerfectGasMixed eos, Range3 r, const float* __restrict__ rho,
+                                                        const float* __restrict__ rhou, const float* __restrict__ E, double* __restrict__ partials)
+{
+    __shared__ double s_red[4];
+    int i, j, k;
+    double v = reduce_identity<kMax>();
+    if (range_cell(r, i, j, k)) {
+        int64_t const id = flat(G, i, j, k);
+        State<ND, float> c;
+        c[0] = rho[id];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) c[1 + d] = rhou[id + G.plane * d];
+        c[ND + 1] = E[id];
+        v = state_rate<ND, GEOM>(G, eos, i, j, k, c);
+    }
+    int const tid = threadIdx.x + kBX * threadIdx.y;
+    v = block_reduce<kMax>(v, s_red, tid, kBX * kBY);
+    if (tid == 0) partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = v;
+}
+
+template <class In, class Out>
+__global__ __launch_bounds__(256) void k_cast(const In* __restrict__ in, Out* __restrict__ out, int64_t n)
+{
+    int64_t const t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) out[t] = (Out)in[t];
 }
 
 // ------------------------------------------------------------ clock
@@ -583,7 +643,7 @@ __device__ __forceinline__ void schedule_next(DevClock* c, double inv)
     c->pending_exit = pending;
 }
 
-__global__ __launch_bounds__(1024) void k_clock_start(DevClock* c, const double* inv)
+__global__ __launch_bounds__(64) void k_clock_start(DevClock* c, const double* inv)
 {
     if (threadIdx.x == 0) schedule_next(c, inv[0]);
 }
@@ -656,19 +716,21 @@ int64_t fused_blocks(const GridDev& G, int nd, FusedLayout* layout)
     return nb;
 }
 
-template <int ND, int GEOM, int RS, int LIM>
-static void launch_march_t(const GridDev& G, const PerfectGas& eos, const StepArrays& A, hipStream_t s)
+template <class Eos, int ND, int GEOM, int RS, int LIM>
+static void launch_march_t(const GridDev& G, const Eos& eos, const StepArraysT<typename Eos::real>& A, hipStream_t s)
 {
     using S = MarchShape<ND>;
     FusedLayout L;
     int64_t const nb = fused_blocks(G, ND, &L);
     if (nb == 0) return;
-    hipLaunchKernelGGL((k_step_march<ND, GEOM, RS, LIM, S::HX, S::HY, S::NT>), dim3((unsigned)nb), dim3(S::NT), 0, s, G, eos, A, L);
+    hipLaunchKernelGGL((k_step_march<Eos, ND, GEOM, RS, LIM, S::HX, S::HY, S::NT>), dim3((unsigned)nb), dim3(S::NT), 0, s, G, eos, A, L);
 }
 
-void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, hipStream_t s)
+template <class Eos>
+static void launch_fused_any(int nd, int geom, int rs, int lim, const GridDev& G, const Eos& eos, const StepArraysT<typename Eos::real>& A,
+                             hipStream_t s)
 {
-#define HCLPP_F4(NDV, GV, RSV, LV) launch_march_t<NDV, GV, RSV, LV>(G, eos, A, s)
+#define HCLPP_F4(NDV, GV, RSV, LV) launch_march_t<Eos, NDV, GV, RSV, LV>(G, eos, A, s)
 #define HCLPP_FL(NDV, GV, RSV)                                     \
     switch (lim) {                                                 \
     case kConstant: HCLPP_F4(NDV, GV, RSV, kConstant); break;      \
@@ -696,6 +758,16 @@ void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, cons
 #undef HCLPP_FG
 }
 
+void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, hipStream_t s)
+{
+    launch_fused_any(nd, geom, rs, lim, G, eos, A, s);
+}
+
+void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGasMixed& eos, const StepArraysF& A, hipStream_t s)
+{
+    launch_fused_any(nd, geom, rs, lim, G, eos, A, s);
+}
+
 GhostShell make_ghost_shell(const GridDev& G, int nd)
 {
     GhostShell B{};
@@ -721,20 +793,64 @@ GhostShell make_ghost_shell(const GridDev& G, int nd)
 
 int64_t ghost_blocks(const GhostShell& B) { return (B.start[B.nbox] + 255) / 256; }
 
-void launch_ghosts_rate(int nd, int geom, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const GhostShell& B, const StepArrays& A,
-                        hipStream_t s)
+template <class Eos>
+static void launch_ghosts_any(int nd, int geom, const GridDev& G, const Eos& eos, const BcSpec& bc, const GhostShell& B,
+                              const StepArraysT<typename Eos::real>& A, hipStream_t s)
 {
     int64_t const nb = ghost_blocks(B);
     if (nb == 0) return;
-#define HCLPP_GR(NDV)                                                                                                            \
-    if (geom == kSpherical) hipLaunchKernelGGL((k_ghosts_rate<NDV, kSpherical>), dim3((unsigned)nb), dim3(256), 0, s, G, eos, bc, B, A); \
-    else hipLaunchKernelGGL((k_ghosts_rate<NDV, kCartesian>), dim3((unsigned)nb), dim3(256), 0, s, G, eos, bc, B, A);
+#define HCLPP_GR(NDV)                                                                                                                    \
+    if (geom == kSpherical) hipLaunchKernelGGL((k_ghosts_rate<Eos, NDV, kSpherical>), dim3((unsigned)nb), dim3(256), 0, s, G, eos, bc, B, A); \
+    else hipLaunchKernelGGL((k_ghosts_rate<Eos, NDV, kCartesian>), dim3((unsigned)nb), dim3(256), 0, s, G, eos, bc, B, A);
     switch (nd) {
     case 1: HCLPP_GR(1); break;
     case 2: HCLPP_GR(2); break;
     default: HCLPP_GR(3); break;
     }
 #undef HCLPP_GR
+}
+
+void launch_ghosts_rate(int nd, int geom, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const GhostShell& B, const StepArrays& A,
+                        hipStream_t s)
+{
+    launch_ghosts_any(nd, geom, G, eos, bc, B, A, s);
+}
+
+void launch_ghosts_rate(int nd, int geom, const GridDev& G, const PerfectGasMixed& eos, const BcSpec& bc, const GhostShell& B,
+                        const StepArraysF& A, hipStream_t s)
+{
+    launch_ghosts_any(nd, geom, G, eos, bc, B, A, s);
+}
+
+void launch_state_rate_f32(int nd, int geom, const GridDev& G, const PerfectGasMixed& eos, const float* rho, const float* rhou, const float* E,
+                           double* partials, double* out, hipStream_t s)
+{
+    Range3 all;
+    for (int d = 0; d < 3; ++d) {
+        all.b[d] = 0;
+        all.e[d] = G.nwg[d];
+    }
+    dim3 const grid = range_grid(all);
+#define HCLPP_SR(NDV)                                                                                                                   \
+    if (geom == kSpherical) hipLaunchKernelGGL((k_state_rate_f32<NDV, kSpherical>), grid, dim3(kBX, kBY), 0, s, G, eos, all, rho, rhou, E, partials); \
+    else hipLaunchKernelGGL((k_state_rate_f32<NDV, kCartesian>), grid, dim3(kBX, kBY), 0, s, G, eos, all, rho, rhou, E, partials);
+    switch (nd) {
+    case 1: HCLPP_SR(1); break;
+    case 2: HCLPP_SR(2); break;
+    default: HCLPP_SR(3); break;
+    }
+#undef HCLPP_SR
+    launch_fold_max(partials, (int64_t)grid.x * grid.y * grid.z, out, s);
+}
+
+void launch_cast(const double* in, float* out, int64_t n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL((k_cast<double, float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n);
+}
+
+void launch_cast(const float* in, double* out, int64_t n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL((k_cast<float, double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n);
 }
 
 void launch_clock_start(DevClock* c, const double* inv, hipStream_t s) { hipLaunchKernelGGL(k_clock_start, dim3(1), dim3(64), 0, s, c, inv); }

@@ -38,17 +38,26 @@ def _dev(a, shape, device="cuda"):
 
 class HydroSolver:
     def __init__(self, grid: Grid, eos: PerfectGas, *, reconstruction: str = "VanLeer", riemann_solver: str = "HLL",
-                 gravity=None, bc=None, bc_order=None, cfl: float = 0.4, nfx: int = 0, fused: bool = True):
+                 gravity=None, bc=None, bc_order=None, cfl: float = 0.4, nfx: int = 0, fused: bool = True,
+                 precision: str = "fp64"):
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32' (FP32 storage with FP64 pressure recovery)")
+        if precision == "fp32" and (not fused or nfx):
+            raise ValueError("FP32 storage is a fused-step mode without passive scalars")
         self.grid = grid
         self.eos = eos
         self.gravity = gravity
         self.cfl = float(cfl)
         self.nfx = nfx
         self.fused = fused and nfx == 0
-        self.ctx = Context(grid, eos, gravity, bc, bc_order, nfx, reconstruction, riemann_solver)
+        self.precision = precision
+        self.ctx = Context(grid, eos, gravity, bc, bc_order, nfx, reconstruction, riemann_solver, precision)
         nd = grid.ndim
         z = grid.zeros
         self._cons = [(z(), z(nd), z()), (z(), z(nd), z())]
+        if precision == "fp32":
+            f = lambda *lead: grid.zeros(*lead, dtype=torch.float32)  # noqa: E731
+            self._cons32 = [(f(), f(nd), f()), (f(), f(nd), f())]
         self._cur = 0
         self.fx = z(max(nfx, 0)) if nfx else None
         self.u = z(nd)
@@ -122,23 +131,48 @@ class HydroSolver:
             if done:
                 return n
 
-    def _run_fused(self, t_end, max_iter, check_every):
+    def _bufs(self):
+        return self._cons32 if self.precision == "fp32" else self._cons
+
+    def _cast(self, src, dst):
+        n = src.numel()
+        fn = lib().hclpp_cast_f64_to_f32 if src.dtype == torch.float64 else lib().hclpp_cast_f32_to_f64
+        check(fn(self.ctx.handle, _ptr(src), _ptr(dst), n))
+
+    def _clock_start(self, t_end, max_iter):
         c = self.ctx
-        nd = self.grid.ndim
+        if self.precision == "fp32":
+            for src, dst in zip(self._cons[self._cur], self._cons32[self._cur]):
+                self._cast(src, dst)
+            r, m, E = self._cons32[self._cur]
+            check(lib().hclpp_clock_start_f32(c.handle, self.t, float(t_end), self.cfl, int(max_iter), _ptr(r), _ptr(m),
+                                              _ptr(E)))
+        else:
+            r, m, E = self._cons[self._cur]
+            check(lib().hclpp_clock_start(c.handle, self.t, float(t_end), self.cfl, int(max_iter), _ptr(r), _ptr(m),
+                                          _ptr(E)))
+
+    def _launch_step(self, a, b):
+        step = lib().hclpp_step_f32 if self.precision == "fp32" else lib().hclpp_step
+        check(step(self.ctx.handle, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(b[0]), _ptr(b[1]), _ptr(b[2])))
+
+    def _sync_back(self):
+        if self.precision == "fp32":
+            for src, dst in zip(self._cons32[self._cur], self._cons[self._cur]):
+                self._cast(src, dst)
+
+    def _run_fused(self, t_end, max_iter, check_every):
         it0 = self.iter
         cur0 = self._cur
-        r, m, E = self._cons[self._cur]
-        check(lib().hclpp_clock_start(c.handle, self.t, float(t_end), self.cfl, int(max_iter), c._chk(r),
-                                      c._chk(m, (nd,)), c._chk(E)))
+        self._clock_start(t_end, max_iter)
+        bufs = self._bufs()
         launched = 0
         while True:
             batch = check_every
             if max_iter - it0 - launched > 0:
                 batch = min(batch, max_iter - it0 - launched)
             for _ in range(batch):
-                a = self._cons[(cur0 + launched) % 2]
-                b = self._cons[(cur0 + launched + 1) % 2]
-                check(lib().hclpp_step(c.handle, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(b[0]), _ptr(b[1]), _ptr(b[2])))
+                self._launch_step(bufs[(cur0 + launched) % 2], bufs[(cur0 + launched + 1) % 2])
                 launched += 1
             clk = self.clock()
             if clk.done or clk.status:
@@ -147,6 +181,7 @@ class HydroSolver:
         self._cur = (cur0 + steps) % 2
         self.t = clk.t
         self.iter = clk.iter
+        self._sync_back()
         self.update_primitives()
         if clk.status:
             raise RuntimeError(f"Time = {clk.t}, iteration = {clk.iter}: detected invalid volumic internal energy")
@@ -154,20 +189,15 @@ class HydroSolver:
 
     def start_clock(self, t_end: float, max_iter: int):
         """Start the device clock on the current state (for callers that drive step_async themselves)."""
-        c = self.ctx
-        nd = self.grid.ndim
-        r, m, E = self._cons[self._cur]
-        check(lib().hclpp_clock_start(c.handle, self.t, float(t_end), self.cfl, int(max_iter), c._chk(r),
-                                      c._chk(m, (nd,)), c._chk(E)))
+        self._clock_start(t_end, max_iter)
         self._launched = 0
         self._cur0 = self._cur
         self._it0 = self.iter
 
     def step_async(self):
         """Enqueue one fused step (no host synchronisation)."""
-        a = self._cons[(self._cur0 + self._launched) % 2]
-        b = self._cons[(self._cur0 + self._launched + 1) % 2]
-        check(lib().hclpp_step(self.ctx.handle, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(b[0]), _ptr(b[1]), _ptr(b[2])))
+        bufs = self._bufs()
+        self._launch_step(bufs[(self._cur0 + self._launched) % 2], bufs[(self._cur0 + self._launched + 1) % 2])
         self._launched += 1
 
     def finish(self):
@@ -177,6 +207,7 @@ class HydroSolver:
         self._cur = (self._cur0 + steps) % 2
         self.t = clk.t
         self.iter = clk.iter
+        self._sync_back()
         if clk.status:
             raise RuntimeError(f"Time = {clk.t}, iteration = {clk.iter}: detected invalid volumic internal energy")
         return clk

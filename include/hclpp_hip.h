@@ -161,6 +161,20 @@ int hclpp_clock_read(hclpp_ctx* ctx, hclpp_clock* out);
 int hclpp_step(hclpp_ctx* ctx, const double* rho, const double* rhou, const double* E, double* rho_new, double* rhou_new,
                double* E_new);
 
+/* ---- FP32 storage (mixed precision) fused step ----
+ * Same step as hclpp_step with the state stored and streamed in FP32 (half
+ * the HBM bytes) and FP32 arithmetic, except the pressure recovery
+ * (gamma-1)(E - |m|^2 / 2 rho), the CFL reduction and the conserved totals,
+ * which are FP64.  The context must be created with precision =
+ * HCLPP_FP32_STORAGE.  Not bit-identical to the FP64 path: the accuracy is
+ * stated as an L1 distance to it. */
+int hclpp_clock_start_f32(hclpp_ctx* ctx, double t0, double t_end, double cfl, int max_iter, const float* rho, const float* rhou,
+                          const float* E);
+int hclpp_step_f32(hclpp_ctx* ctx, const float* rho, const float* rhou, const float* E, float* rho_new, float* rhou_new, float* E_new);
+/* element-wise conversions between FP64 and FP32 device arrays of n values */
+int hclpp_cast_f64_to_f32(hclpp_ctx* ctx, const double* in, float* out, long long n);
+int hclpp_cast_f32_to_f64(hclpp_ctx* ctx, const float* in, double* out, long long n);
+
 /* ---- instrumentation (no reference counterpart; for benchmarks) ----
  * Between begin and end, each hclpp_step records a HIP event pair around its
  * fused kernel on the context's stream (at most max_steps pairs).  end

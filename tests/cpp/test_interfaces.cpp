@@ -1,0 +1,211 @@
+// test_interfaces.cpp -- the C++ host side (include/hclpp/hydro.hpp) driven the way the
+// reference's tests and main loop drive the operator interfaces, checked against the oracle.
+// Built by __graft_entry__.build(); run by tests/test_gpu_cpp_interfaces.py on a GPU.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hclpp/hydro.hpp"
+#include "hclpp_oracle.h"
+
+using namespace hclpp::hip;
+
+static int g_fail = 0;
+static int g_pass = 0;
+#define EXPECT(cond, ...)                                                    \
+    do {                                                                     \
+        if (cond) {                                                          \
+            ++g_pass;                                                        \
+        } else {                                                             \
+            ++g_fail;                                                        \
+            std::printf("FAIL %s:%d: %s ", __FILE__, __LINE__, #cond);      \
+            std::printf(__VA_ARGS__);                                        \
+            std::printf("\n");                                               \
+        }                                                                    \
+    } while (0)
+
+template <class F>
+static bool throws_runtime(F f, char const* needle)
+{
+    try {
+        f();
+    } catch (std::runtime_error const& e) {
+        return std::strstr(e.what(), needle) != nullptr;
+    }
+    return false;
+}
+
+// tests/conversions.cpp:17-103 on the device through conv_prim_to_cons / conv_cons_to_prim
+static void test_conversions()
+{
+    int const n = 10;
+    PerfectGas const eos(2, 1);
+    for (int nd = 1; nd <= 3; ++nd) {
+        Grid const grid(nd, {n, n, n}, 0, Geometry::Geom_cartesian, {0, 0, 0}, {1, 1, 1});
+        auto const& w = grid.Nx_local_wg;
+        KV_double_3d rho("rho", w[0], w[1], w[2]), P("P", w[0], w[1], w[2]), E("E", w[0], w[1], w[2]);
+        KV_double_4d u("u", w[0], w[1], w[2], nd), rhou("rhou", w[0], w[1], w[2], nd);
+        size_t const cells = rho.size();
+        rho.copy_from_host(std::vector<double>(cells, 2.));
+        u.copy_from_host(std::vector<double>(cells * nd, -1.));
+        P.copy_from_host(std::vector<double>(cells, 10.));
+        conv_prim_to_cons(grid.range.all_ghosts(), eos, grid, rho, u, P, rhou, E);
+        double prim[5] = {2., -1., -1., -1., 10.}, cons[5];
+        prim[1 + nd] = 10.;
+        or_to_cons(nd, 2., 1., prim, cons);
+        auto const hr = rhou.to_host();
+        auto const he = E.to_host();
+        bool ok = true;
+        for (size_t c = 0; c < cells; ++c) {
+            ok = ok && he[c] == cons[1 + nd];
+            for (int d = 0; d < nd; ++d) ok = ok && hr[c + cells * d] == cons[1 + d];
+        }
+        EXPECT(ok, "prim->cons nd=%d", nd);
+        rhou.copy_from_host(std::vector<double>(cells * nd, -2.));
+        E.copy_from_host(std::vector<double>(cells, 10.));
+        conv_cons_to_prim(grid.range.all_ghosts(), eos, grid, rho, rhou, E, u, P);
+        double c2[5] = {2., -2., -2., -2., 10.}, p2[5];
+        c2[1 + nd] = 10.;
+        or_to_prim(nd, 2., 1., c2, p2);
+        auto const hu = u.to_host();
+        auto const hp = P.to_host();
+        ok = true;
+        for (size_t c = 0; c < cells; ++c) {
+            ok = ok && hp[c] == p2[1 + nd];
+            for (int d = 0; d < nd; ++d) ok = ok && hu[c + cells * d] == p2[1 + d];
+        }
+        EXPECT(ok, "cons->prim nd=%d", nd);
+    }
+}
+
+static void test_errors()
+{
+    EXPECT(throws_runtime([] { factory_face_reconstruction("Superbee"); }, "Unknown face reconstruction algorithm: Superbee."), "factory");
+    EXPECT(throws_runtime([] { factory_godunov_scheme<UniformGravity>("Roe", PerfectGas(1.4, 1)); }, "Invalid riemann solver: Roe."), "factory");
+    bool dom = false;
+    try {
+        PerfectGas(1.0, 1.0);
+    } catch (std::domain_error const&) {
+        dom = true;
+    }
+    EXPECT(dom, "PerfectGas(1, 1) must throw std::domain_error");
+    Range const r({0, 0, 0}, {4, 4, 4}, 2, 3);
+    EXPECT(throws_runtime([&] { (void)r.with_ghosts(3); }, "NgEff > Ng"), "with_ghosts");
+}
+
+// KH-like random state: the reference main loop (main.cpp:354-531) written with the interface
+// classes, against the oracle's driver; every conservative array must be bit-identical.
+static void test_operator_loop()
+{
+    int const nd = 2, n = 40;
+    double const cfl = 0.4, gamma = 1.4;
+    Grid const grid(nd, {n, n, 1}, 2, Geometry::Geom_cartesian, {0, 0, 0}, {1, 1, 1});
+    PerfectGas const eos(gamma, 1.);
+    UniformGravity const g({0.0, -0.1, 0.0});
+    auto const& w = grid.Nx_local_wg;
+    size_t const cells = static_cast<size_t>(w[0]) * w[1] * w[2];
+    std::vector<double> hrho(cells, 0.), hu(cells * nd, 0.), hP(cells, 0.);
+    for (int j = 2; j < n + 2; ++j)
+        for (int i = 2; i < n + 2; ++i) {
+            size_t const c = i + w[0] * j;
+            double const x = (i - 1.5) / n, y = (j - 1.5) / n;
+            hrho[c] = std::abs(y - 0.5) < 0.25 ? 2.0 : 1.0;
+            hu[c] = std::abs(y - 0.5) < 0.25 ? 0.5 : -0.5;
+            hu[c + cells] = 0.01 * std::sin(4 * M_PI * x) * std::sin(7.0 * x + 3.0 * y);
+            hP[c] = 2.5;
+        }
+    // oracle driver
+    or_config oc {};
+    oc.ndim = nd;
+    oc.geom = OR_GEOM_CARTESIAN;
+    oc.limiter = OR_LIM_VANLEER;
+    oc.riemann = OR_RS_HLLC;
+    oc.gravity = OR_GRAV_UNIFORM;
+    oc.ng = 2;
+    oc.n[0] = n;
+    oc.n[1] = n;
+    oc.n[2] = 1;
+    for (int f = 0; f < 6; ++f) {
+        oc.bc[f] = f < 2 ? OR_BC_PERIODIC : OR_BC_REFLEXIVE;
+        oc.bc_order[f] = f;
+    }
+    oc.gamma = gamma;
+    oc.mmw = 1.;
+    oc.g[1] = -0.1;
+    for (int d = 0; d < 3; ++d) oc.x[d] = grid.x[d].data();
+    or_ctx* o = or_create(&oc);
+    or_init_from_prim(o, hrho.data(), hu.data(), hP.data(), nullptr);
+    int const nsteps = 8;
+    for (int s = 0; s < nsteps; ++s) {
+        double dt;
+        int ex;
+        or_step(o, cfl, 1e30, nsteps, &dt, &ex);
+    }
+    // device: the reference sequence through the interfaces
+    KV_double_3d rho("rho", w[0], w[1], w[2]), P("P", w[0], w[1], w[2]), E("E", w[0], w[1], w[2]);
+    KV_double_4d u("u", w[0], w[1], w[2], nd), rhou("rhou", w[0], w[1], w[2], nd), fx("fx", w[0], w[1], w[2], 0);
+    KV_double_3d rho_new("rho_new", w[0], w[1], w[2]), E_new("E_new", w[0], w[1], w[2]);
+    KV_double_4d rhou_new("rhou_new", w[0], w[1], w[2], nd), fx_new("fx_new", w[0], w[1], w[2], 0);
+    KV_double_5d rho_rec("rho_rec", w[0], w[1], w[2], 2, nd), E_rec("E_rec", w[0], w[1], w[2], 2, nd), P_rec("P_rec", w[0], w[1], w[2], 2, nd);
+    KV_double_6d rhou_rec("rhou_rec", w[0], w[1], w[2], 2, nd, nd), u_rec("u_rec", w[0], w[1], w[2], 2, nd, nd),
+            fx_rec("fx_rec", w[0], w[1], w[2], 2, nd, 0);
+    rho.copy_from_host(hrho);
+    u.copy_from_host(hu);
+    P.copy_from_host(hP);
+    DistributedBoundaryCondition const bcs(grid, {"Periodic", "Periodic", "Reflexive", "Reflexive"});
+    auto reconstruction = std::make_unique<MUSCLHancockHydroReconstruction<UniformGravity>>(
+            factory_face_reconstruction("VanLeer"), std::make_unique<HancockExtrapolationReconstruction<UniformGravity>>(eos), eos, P_rec, u_rec);
+    auto godunov = factory_godunov_scheme<UniformGravity>("HLLC", eos);
+    conv_prim_to_cons(grid.range.no_ghosts(), eos, grid, rho, u, P, rhou, E);  // main.cpp:305
+    bcs(grid, rho, rhou, E, fx);                                                 // :307
+    conv_cons_to_prim(grid.range.all_ghosts(), eos, grid, rho, rhou, E, u, P);  // :311
+    for (int s = 0; s < nsteps; ++s) {
+        double const dt = cfl * time_step(grid.range.all_ghosts(), eos, grid, rho, u, P);  // :355
+        double const emin = minimum_internal_energy(grid.range.no_ghosts(), grid, rho, rhou, E);
+        EXPECT(emin > 0, "internal energy");
+        reconstruction->execute(grid.range.with_ghosts(1), grid, g, dt / 2, rho, u, P, fx, rho_rec, rhou_rec, E_rec, fx_rec);
+        godunov->execute(grid.range.no_ghosts(), grid, g, dt, rho, rhou, E, fx, rho_rec, rhou_rec, E_rec, fx_rec, rho_new, rhou_new, E_new, fx_new);
+        bcs(grid, rho_new, rhou_new, E_new, fx_new);
+        std::swap(rho, rho_new);
+        std::swap(rhou, rhou_new);
+        std::swap(E, E_new);
+        conv_cons_to_prim(grid.range.all_ghosts(), eos, grid, rho, rhou, E, u, P);
+    }
+    auto const r = rho.to_host();
+    auto const m = rhou.to_host();
+    auto const e = E.to_host();
+    EXPECT(std::memcmp(r.data(), or_state(o, 0), r.size() * 8) == 0, "rho differs from the oracle");
+    EXPECT(std::memcmp(m.data(), or_state(o, 1), m.size() * 8) == 0, "rhou differs from the oracle");
+    EXPECT(std::memcmp(e.data(), or_state(o, 2), e.size() * 8) == 0, "E differs from the oracle");
+
+    // the fused driver on the same problem
+    HydroDriver<UniformGravity> drv(grid, eos, g, "VanLeer", "HLLC", bcs, cfl);
+    rho.copy_from_host(hrho);
+    u.copy_from_host(hu);
+    P.copy_from_host(hP);
+    drv.init_from_prim(rho, u, P);
+    int const steps = drv.run(1e30, nsteps);
+    EXPECT(steps == nsteps, "steps %d", steps);
+    EXPECT(std::memcmp(drv.rho().to_host().data(), or_state(o, 0), r.size() * 8) == 0, "fused rho differs from the oracle");
+    EXPECT(std::memcmp(drv.rhou().to_host().data(), or_state(o, 1), m.size() * 8) == 0, "fused rhou differs from the oracle");
+    EXPECT(std::memcmp(drv.E().to_host().data(), or_state(o, 2), e.size() * 8) == 0, "fused E differs from the oracle");
+    EXPECT(drv.time() == or_time(o), "time %.17g vs %.17g", drv.time(), or_time(o));
+    or_destroy(o);
+}
+
+int main()
+{
+    try {
+        test_errors();
+        test_conversions();
+        test_operator_loop();
+    } catch (std::exception const& e) {
+        std::printf("FAIL: exception %s\n", e.what());
+        return 2;
+    }
+    std::printf("%d passed, %d failed\n", g_pass, g_fail);
+    return g_fail == 0 ? 0 : 1;
+}

@@ -17,7 +17,7 @@ def make_oracle(s: setups.Setup, **kw) -> orc.Oracle:
     return o
 
 
-def make_solver(s: setups.Setup, fused=True, **kw):
+def make_solver(s: setups.Setup, fused=True, precision="fp64", **kw):
     from heraclespp_amd.hydro import PointMassGravity
     from heraclespp_amd.solver import HydroSolver
 
@@ -28,7 +28,7 @@ def make_solver(s: setups.Setup, fused=True, **kw):
         g = UniformGravity(s.gravity)
     return HydroSolver(grid, PerfectGas(s.gamma, 1.0), reconstruction=kw.get("limiter", s.reconstruction),
                        riemann_solver=kw.get("riemann", s.riemann_solver), gravity=g, bc=s.bc, cfl=s.cfl, fused=fused,
-                       nfx=kw.get("nfx", 0), bc_order=kw.get("bc_order"))
+                       nfx=kw.get("nfx", 0), bc_order=kw.get("bc_order"), precision=precision)
 
 
 def run_oracle(o: orc.Oracle, s: setups.Setup, nsteps: int, t_end=None):
