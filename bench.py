@@ -95,6 +95,71 @@ def cpu_baseline(name: str, seconds: float):
     }
 
 
+def adjoint_block(n: int, steps: int, every: int, fd_dirs: int):
+    """Differentiable-Euler workload: forward rollout, checkpointed backward and reverse-kernel cost per step,
+    plus a central-difference check of the gradient on a few seeded unit directions."""
+    import numpy as np
+    import torch
+
+    from heraclespp_amd.adjoint import diff_euler_problem
+
+    prob = diff_euler_problem(n, steps, seed=0)
+    prob.gradient(every=every)  # warm-up: allocations, first launches
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
+
+    g = torch.tensor(prob.gamma, dtype=torch.float64)
+    with torch.no_grad():
+        t_fwd, _ = timed(lambda: prob.loss(prob.rho0, g, every))
+    x = prob.rho0.clone().requires_grad_(True)
+    gg = g.clone().requires_grad_(True)
+    t_fwd_ckpt, L = timed(lambda: prob.loss(x, gg, every))
+    t_bwd, _ = timed(lambda: L.backward())
+    # reverse kernels alone: `steps` reverse steps on one state
+    eng = prob.spec.engine(prob.gamma)
+    U = eng.load(*prob.cons(prob.rho0, g))
+    lam, lam2 = eng.new_state(), eng.new_state()
+    lam[0].normal_()
+    dg = torch.zeros(1, dtype=torch.float64, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    eng.vjp(U, lam, lam2, dg, prob.dts[0])
+    e0.record()
+    for k in range(steps):
+        eng.vjp(U, lam, lam2, dg, prob.dts[k])
+    e1.record()
+    torch.cuda.synchronize()
+    t_rev = e0.elapsed_time(e1) / 1e3
+    rng = np.random.Generator(np.random.Philox(1))
+    worst = 0.0
+    for _ in range(fd_dirs):
+        v = torch.from_numpy(rng.standard_normal((n, n))).cuda()
+        v = v / torch.linalg.norm(v)
+        fd = prob.fd(v, 0.0, 1e-6)
+        ad = float(torch.sum(x.grad * v))
+        worst = max(worst, abs(fd - ad) / max(abs(fd), 1e-300))
+    fdg = prob.fd(torch.zeros_like(prob.rho0), 1.0, 1e-6)
+    relg = abs(fdg - float(gg.grad)) / max(abs(fdg), 1e-300)
+    f = t_fwd / steps * 1e3
+    return {
+        "workload": f"differentiable Euler {n}x{n} periodic, VanAlbada+HLLC, {steps} steps, checkpoint every {every}",
+        "forward_ms_per_step": round(f, 5),
+        "forward_with_checkpoints_ms_per_step": round(t_fwd_ckpt / steps * 1e3, 5),
+        "backward_ms_per_step": round(t_bwd / steps * 1e3, 5),
+        "reverse_kernels_ms_per_step": round(t_rev / steps * 1e3, 5),
+        "ratio_reverse_to_forward": round(t_rev / t_fwd, 3),
+        "ratio_backward_to_forward": round(t_bwd / t_fwd, 3),
+        "fd_directions": fd_dirs,
+        "fd_rel_err_max": float(f"{worst:.3e}"),
+        "fd_rel_err_gamma": float(f"{relg:.3e}"),
+    }
+
+
 def calibrate_copy(nbytes: int):
     """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
     import torch
@@ -118,6 +183,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-Euler adjoint measurement")
+    ap.add_argument("--adjoint-n", type=int, default=1024)
+    ap.add_argument("--adjoint-steps", type=int, default=400)
+    ap.add_argument("--adjoint-fd-dirs", type=int, default=4)
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -206,9 +275,13 @@ def main():
             "bytes_per_launch": bytes_alg,
         },
         "cpu_baseline": None,
+        "adjoint": None,
     }
     del solver
     torch.cuda.empty_cache()
+    if not a.no_adjoint:
+        line["adjoint"] = adjoint_block(a.adjoint_n, a.adjoint_steps, 20, a.adjoint_fd_dirs)
+        torch.cuda.empty_cache()
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_seconds)
     print(json.dumps(line), flush=True)

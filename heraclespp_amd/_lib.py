@@ -32,8 +32,8 @@ EXPORTS = [
     "hclpp_conv_prim_to_cons", "hclpp_conv_cons_to_prim", "hclpp_face_reconstruction", "hclpp_hancock_extrapolation",
     "hclpp_hydro_reconstruction", "hclpp_godunov", "hclpp_time_step", "hclpp_min_internal_energy", "hclpp_integrate",
     "hclpp_fill_ghosts", "hclpp_clock_start", "hclpp_clock_read", "hclpp_step", "hclpp_clock_start_f32", "hclpp_step_f32",
-    "hclpp_cast_f64_to_f32", "hclpp_cast_f32_to_f64", "hclpp_profile_begin",
-    "hclpp_profile_end",
+    "hclpp_cast_f64_to_f32", "hclpp_cast_f32_to_f64", "hclpp_step_dt", "hclpp_clock_record_dt", "hclpp_step_adjoint",
+    "hclpp_profile_begin", "hclpp_profile_end",
 ]
 
 
@@ -122,6 +122,9 @@ def lib() -> ctypes.CDLL:
         "hclpp_step_f32": [vp, dp, dp, dp, dp, dp, dp],
         "hclpp_cast_f64_to_f32": [vp, dp, dp, ctypes.c_longlong],
         "hclpp_cast_f32_to_f64": [vp, dp, dp, ctypes.c_longlong],
+        "hclpp_step_dt": [vp, ctypes.c_double, dp, dp, dp, dp, dp, dp],
+        "hclpp_clock_record_dt": [vp, dp, ip],
+        "hclpp_step_adjoint": [vp, ctypes.c_double] + [dp] * 10,
         "hclpp_profile_begin": [vp, ip],
         "hclpp_profile_end": [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_int)],
     }

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint_step.hpp"
 #include "fused.hpp"
 #include "hclpp_hip.h"
 #include "launch.hpp"
@@ -46,6 +47,9 @@ struct hclpp_ctx {
     int64_t fx_old_cap = 0;
     GhostShell shell{};
     int64_t fused_nb = 0;
+    double* adj_prim = nullptr;     // reverse step: primitives (NV planes)
+    double* adj_face = nullptr;     // reverse step: face slots (2 ND NV planes)
+    double* adj_partials = nullptr; // reverse step: dgamma partials
     std::vector<hipEvent_t> prof_events;  // pairs (start, stop) per timed fused launch
     int prof_max = 0;
     int prof_count = 0;
@@ -297,6 +301,9 @@ int hclpp_ctx_destroy(hclpp_ctx* c)
     if (c->clock) (void)hipFree(c->clock);
     if (c->host_clock) (void)hipHostFree(c->host_clock);
     if (c->fx_old) (void)hipFree(c->fx_old);
+    if (c->adj_prim) (void)hipFree(c->adj_prim);
+    if (c->adj_face) (void)hipFree(c->adj_face);
+    if (c->adj_partials) (void)hipFree(c->adj_partials);
     for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
     delete c;
     return HCLPP_OK;
@@ -622,6 +629,53 @@ int hclpp_step(hclpp_ctx* c, const double* rho, const double* rhou, const double
     HCLPP_LAUNCHED("ghost fill");
     launch_clock_advance(nd, c->clock, c->partials, c->fused_nb, c->ghost_partials, c->ghost_cap, c->stream);
     HCLPP_LAUNCHED("clock advance");
+    return HCLPP_OK;
+}
+
+int hclpp_step_dt(hclpp_ctx* c, double dt, const double* rho, const double* rhou, const double* E, double* rho_new, double* rhou_new,
+                  double* E_new)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (!std::isfinite(dt) || !(dt > 0)) return fail(HCLPP_ERR_ARG, "dt must be finite and > 0");
+    if (c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "hclpp_step_dt takes FP64 storage");
+    launch_clock_set_dt(c->clock, dt, c->stream);
+    HCLPP_LAUNCHED("clock dt");
+    return hclpp_step(c, rho, rhou, E, rho_new, rhou_new, E_new);
+}
+
+int hclpp_clock_record_dt(hclpp_ctx* c, double* log, int cap)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (log && cap < 0) return fail(HCLPP_ERR_ARG, "cap < 0");
+    launch_clock_set_log(c->clock, log, cap, c->stream);
+    HCLPP_LAUNCHED("clock dt log");
+    return HCLPP_OK;
+}
+
+int hclpp_step_adjoint(hclpp_ctx* c, double dt, const double* rho, const double* rhou, const double* E, const double* lrho_new,
+                       const double* lrhou_new, const double* lE_new, double* lrho, double* lrhou, double* lE, double* dgamma)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step takes FP64 storage");
+    if (c->cfg.geom != HCLPP_GEOM_CARTESIAN) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step supports Cartesian geometry only");
+    if (c->cfg.riemann != HCLPP_RS_HLL && c->cfg.riemann != HCLPP_RS_HLLC)
+        return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step supports the HLL and HLLC solvers only");
+    if (c->cfg.nfx > 0) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step has no passive scalars");
+    if (c->cfg.ng < 2 || c->cfg.ng > adj::kMaxGhosts) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step needs 2 <= ng <= 4");
+    if (!std::isfinite(dt) || !(dt > 0)) return fail(HCLPP_ERR_ARG, "dt must be finite and > 0");
+    if (!rho || !rhou || !E || !lrho_new || !lrhou_new || !lE_new || !lrho || !lrhou || !lE || !dgamma)
+        return fail(HCLPP_ERR_ARG, "null array");
+    if (lrho == lrho_new || lrhou == lrhou_new || lE == lE_new) return fail(HCLPP_ERR_ARG, "input and output adjoints must differ");
+    int const nd = c->cfg.ndim;
+    int const nv = nd + 2;
+    if (!c->adj_prim) {
+        HCLPP_HIP(hipMalloc(&c->adj_prim, sizeof(double) * c->G.plane * nv));
+        HCLPP_HIP(hipMalloc(&c->adj_face, sizeof(double) * c->G.plane * 2 * nd * nv));
+        HCLPP_HIP(hipMalloc(&c->adj_partials, sizeof(double) * adjoint_partials(c->G, nd)));
+    }
+    adj::AdjArrays A{rho, rhou, E, c->adj_prim, c->adj_face, lrho_new, lrhou_new, lE_new, lrho, lrhou, lE, dt};
+    launch_step_adjoint(nd, c->cfg.riemann, c->cfg.limiter, c->G, c->eos, c->bc, A, c->adj_partials, dgamma, c->stream);
+    HCLPP_LAUNCHED("reverse step");
     return HCLPP_OK;
 }
 

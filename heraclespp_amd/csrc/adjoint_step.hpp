@@ -1,0 +1,446 @@
+// adjoint_step.hpp -- per-cell and per-face bodies of the reverse-mode hydro step.
+//
+// The forward step (Cartesian) is
+//   Ug = ghost_fill(U);  p = prim(Ug)                         (all cells)
+//   pf[s][d] = p -/+ dl/2 * slope_d(p)                        (cells within 1 of the interior)
+//   Uf[s][d] = cons(pf[s][d]) + dth/dV sum_e (F_e(pf[0][e]) dS - F_e(pf[1][e]) dS+) + gravity(dth)
+//   U'(c) = U(c) + dt/dV sum_d (F*(face c-1/2) dS - F*(face c+1/2) dS+) + gravity(dt)
+// with dth = dt / 2 and F* the Riemann flux of (Uf[1][d] of the left cell, Uf[0][d] of the right one).
+// Given the adjoint lambda' of U' on the interior, the reverse step runs five passes; each
+// writes every output element exactly once (no atomics, results are deterministic):
+//   prim   (all cells)          p = prim(Ug)
+//   faces  (ring-1 cells)       Uf, stored in the face-slot buffer
+//   flux   (faces touching the interior, one pass per dim)
+//                               lambda_F = dt/dV (lambda'(right) dS - lambda'(left) dS+), Riemann VJP,
+//                               the adjoints of the two face states overwrite their slots
+//   hancock (ring-1 cells)      adjoint of the Hancock predictor: face-state adjoints -> face-prim
+//                               adjoints, overwriting the slots
+//   gather (interior cells)     face-prim adjoints -> cell-prim adjoints through the slope
+//                               limiter (gathering the +-1 neighbours' terms), prim -> cons VJP,
+//                               then the transposed ghost fill (every ghost cell whose value was
+//                               copied from this cell adds its adjoint back, sign-flipped where
+//                               the fill negated the momentum), the identity and gravity terms.
+// Slot (side s, dim d) of a cell holds NV values at planes v + NV * (s + 2 d).
+// Every pass also returns the cell's or face's contribution to dL/dgamma.
+#pragma once
+
+#include "adjoint.hpp"
+#include "grid.hpp"
+#include "launch.hpp"
+
+namespace hclpp::adj {
+
+constexpr int kMaxGhosts = 4;
+constexpr int kMaxPre = 1 + 2 * kMaxGhosts;
+
+struct AdjArrays {
+    const double* rho;  // step input, ghosts filled
+    const double* rhou;
+    const double* E;
+    double* prim;       // NV planes
+    double* face;       // 2 ND NV planes (face slots)
+    const double* lrho_new;  // adjoint of the step output (interior cells read)
+    const double* lrhou_new;
+    const double* lE_new;
+    double* lrho;       // adjoint of the step input (interior cells written)
+    double* lrhou;
+    double* lE;
+    double dt;
+};
+
+__host__ __device__ __forceinline__ bool inside(const GridDev& G, int d, int a) { return a >= G.ngh[d] && a < G.ngh[d] + G.n[d]; }
+
+template <int ND>
+__host__ __device__ __forceinline__ bool ring1(const GridDev& G, const int* c)
+{
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+        if (c[d] < G.ngh[d] - 1 || c[d] > G.ngh[d] + G.n[d]) return false;
+    return true;
+}
+
+template <int ND>
+__host__ __device__ __forceinline__ S<ND> load_planes(const double* a, int64_t id, int64_t pl, int base)
+{
+    S<ND> q;
+#pragma unroll
+    for (int v = 0; v < ND + 2; ++v) q[v] = a[id + pl * (base + v)];
+    return q;
+}
+
+template <int ND>
+__host__ __device__ __forceinline__ void store_planes(double* a, int64_t id, int64_t pl, int base, const S<ND>& q)
+{
+#pragma unroll
+    for (int v = 0; v < ND + 2; ++v) a[id + pl * (base + v)] = q[v];
+}
+
+template <int ND>
+__host__ __device__ __forceinline__ S<ND> load_cons(const double* rho, const double* rhou, const double* E, int64_t id, int64_t pl)
+{
+    S<ND> q;
+    q[0] = rho[id];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) q[1 + d] = rhou[id + pl * d];
+    q[ND + 1] = E[id];
+    return q;
+}
+
+__host__ __device__ __forceinline__ constexpr int slot(int nv, int s, int d) { return nv * (s + 2 * d); }
+
+__host__ __device__ __forceinline__ int64_t shifted(const GridDev& G, const int* c, int d, int by)
+{
+    int q[3] = {c[0], c[1], c[2]};
+    q[d] += by;
+    return flat(G, q[0], q[1], q[2]);
+}
+
+// Face primitives of cell c (limited_linear_reconstruction.hpp:46-64), same expressions as the forward.
+template <int ND, int LIM>
+__host__ __device__ __forceinline__ void face_prims(const GridDev& G, const double* prim, const int* c, const S<ND>& p, S<ND> (&pf)[2][ND])
+{
+    int64_t const pl = G.plane;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        S<ND> const pm = load_planes<ND>(prim, shifted(G, c, d, -1), pl, 0);
+        S<ND> const pp = load_planes<ND>(prim, shifted(G, c, d, +1), pl, 0);
+        double const dl = G.dx[d][c[d]];
+        double const dl_m = G.dx[d][c[d] - 1];
+        double const dl_p = G.dx[d][c[d] + 1];
+#pragma unroll
+        for (int v = 0; v < ND + 2; ++v) {
+            double const slope = limited_slope_t<LIM>((pp[v] - p[v]) / ((dl + dl_p) / 2), (p[v] - pm[v]) / ((dl_m + dl) / 2));
+            pf[0][d][v] = p[v] - ((dl / 2) * slope);
+            pf[1][d][v] = p[v] + ((dl / 2) * slope);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- pass 1: primitives of every cell
+template <int ND>
+__host__ __device__ __forceinline__ void cell_prim(const GridDev& G, const PerfectGas& eos, const AdjArrays& A, int i, int j, int k)
+{
+    int64_t const id = flat(G, i, j, k);
+    store_planes<ND>(A.prim, id, G.plane, 0, cons_to_prim(load_cons<ND>(A.rho, A.rhou, A.E, id, G.plane), eos));
+}
+
+// ---------------------------------------------------------------- pass 2: Hancock face states (ring-1 cells)
+// extrapolation_reconstruction.hpp:90-325 with dt/2, in the forward's accumulation order.
+template <int ND, int LIM>
+__host__ __device__ __forceinline__ void cell_faces(const GridDev& G, const PerfectGas& eos, const AdjArrays& A, int i, int j, int k)
+{
+    int const c[3] = {i, j, k};
+    int64_t const id = flat(G, i, j, k);
+    int64_t const pl = G.plane;
+    S<ND> const p = load_planes<ND>(A.prim, id, pl, 0);
+    S<ND> pf[2][ND];
+    face_prims<ND, LIM>(G, A.prim, c, p, pf);
+    S<ND> fs[2][ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) fs[s][d] = prim_to_cons(pf[s][d], eos);
+    double rho_old[2][ND], rhou_old[2][ND][ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            rho_old[s][d] = fs[s][d][0];
+#pragma unroll
+            for (int v = 0; v < ND; ++v) rhou_old[s][d][v] = fs[s][d][1 + v];
+        }
+    double const dth = A.dt / 2;
+#pragma unroll
+    for (int dim = 0; dim < ND; ++dim) {
+        S<ND> const fL = flux_of_prim(pf[0][dim], dim, eos);
+        S<ND> const fR = flux_of_prim(pf[1][dim], dim, eos);
+        double const dtodv = dth / cell_volume<ND, kCartesian>(G, i, j, k);
+        double const dS = face_area<ND, kCartesian>(G, dim, i, j, k);
+        double const dSp = face_area<ND, kCartesian>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2));
+#pragma unroll
+        for (int pos = 0; pos < ND; ++pos)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int v = 0; v < ND + 2; ++v) fs[s][pos][v] += dtodv * ((fL[v] * dS) - (fR[v] * dSp));
+        double const g = gravity(G, i, dim);
+#pragma unroll
+        for (int pos = 0; pos < ND; ++pos)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                fs[s][pos][1 + dim] += dth * g * rho_old[s][pos];
+                fs[s][pos][ND + 1] += dth * g * rhou_old[s][pos][dim];
+            }
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) store_planes<ND>(A.face, id, pl, slot(ND + 2, s, d), fs[s][d]);
+}
+
+// ---------------------------------------------------------------- pass 3: Riemann flux adjoint (one face)
+// Face between cell c - e_d and cell c, with c_d in [ng, ng + n] and c interior along the other dims.
+// The face direction is a template parameter: the Riemann VJP indexes state components by it,
+// and a run-time index would push the register arrays to scratch memory.
+template <int ND, int RS, int d>
+__host__ __device__ __forceinline__ double face_flux(const GridDev& G, const PerfectGas& eos, const AdjArrays& A, int i, int j, int k)
+{
+    constexpr int NV = ND + 2;
+    int const c[3] = {i, j, k};
+    int64_t const id = flat(G, i, j, k);
+    int64_t const idm = shifted(G, c, d, -1);
+    int64_t const pl = G.plane;
+    S<ND> const cL = load_planes<ND>(A.face, idm, pl, slot(NV, 1, d));
+    S<ND> const cR = load_planes<ND>(A.face, id, pl, slot(NV, 0, d));
+    S<ND> lF = zero<ND>();
+    // godunov_scheme.hpp:110-292: U'(c) += dt/dV F dS;  U'(c - e_d) -= dt/dV F dS+
+    if (inside(G, d, c[d])) {
+        double const w = (A.dt / cell_volume<ND, kCartesian>(G, i, j, k)) * face_area<ND, kCartesian>(G, d, i, j, k);
+        S<ND> const l = load_cons<ND>(A.lrho_new, A.lrhou_new, A.lE_new, id, pl);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lF[v] += w * l[v];
+    }
+    if (inside(G, d, c[d] - 1)) {
+        int const im = i - (d == 0), jm = j - (d == 1), km = k - (d == 2);
+        double const w = (A.dt / cell_volume<ND, kCartesian>(G, im, jm, km)) * face_area<ND, kCartesian>(G, d, i, j, k);
+        S<ND> const l = load_cons<ND>(A.lrho_new, A.lrhou_new, A.lE_new, idm, pl);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lF[v] -= w * l[v];
+    }
+    S<ND> lcL = zero<ND>(), lcR = zero<ND>();
+    double lg = 0;
+    riemann_vjp<RS>(cL, cR, d, eos, lF, lcL, lcR, lg);
+    store_planes<ND>(A.face, idm, pl, slot(NV, 1, d), lcL);
+    store_planes<ND>(A.face, id, pl, slot(NV, 0, d), lcR);
+    return lg;
+}
+
+// Slot (s, d) of ring-1 cell c carries a flux adjoint iff its face touches an interior cell.
+template <int ND>
+__host__ __device__ __forceinline__ bool slot_live(const GridDev& G, const int* c, int s, int d)
+{
+#pragma unroll
+    for (int e = 0; e < ND; ++e)
+        if (e != d && !inside(G, e, c[e])) return false;
+    return inside(G, d, c[d]) || inside(G, d, c[d] + (s ? 1 : -1));
+}
+
+// ---------------------------------------------------------------- pass 4: Hancock adjoint (ring-1 cells)
+template <int ND, int LIM>
+__host__ __device__ __forceinline__ double cell_hancock(const GridDev& G, const PerfectGas& eos, const AdjArrays& A, int i, int j, int k)
+{
+    constexpr int NV = ND + 2;
+    int const c[3] = {i, j, k};
+    int64_t const id = flat(G, i, j, k);
+    int64_t const pl = G.plane;
+    S<ND> lam[2][ND];
+    bool any = false;
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            bool const live = slot_live<ND>(G, c, s, d);
+            any = any || live;
+            lam[s][d] = live ? load_planes<ND>(A.face, id, pl, slot(NV, s, d)) : zero<ND>();
+        }
+    S<ND> lpf[2][ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) lpf[s][d] = zero<ND>();
+    double lg = 0;
+    if (any) {
+        S<ND> const p = load_planes<ND>(A.prim, id, pl, 0);
+        S<ND> pf[2][ND];
+        face_prims<ND, LIM>(G, A.prim, c, p, pf);
+        double const dth = A.dt / 2;
+        S<ND> lD = zero<ND>();
+#pragma unroll
+        for (int pos = 0; pos < ND; ++pos)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                S<ND> lc = lam[s][pos];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) lD[v] += lam[s][pos][v];
+                // gravity on the face's own conservative state
+#pragma unroll
+                for (int dim = 0; dim < ND; ++dim) {
+                    double const g = gravity(G, i, dim);
+                    lc[0] += dth * g * lam[s][pos][1 + dim];
+                    lc[1 + dim] += dth * g * lam[s][pos][ND + 1];
+                }
+                cons_vjp(pf[s][pos], eos.gamma, lc, lpf[s][pos], lg);
+            }
+        double const dtodv = dth / cell_volume<ND, kCartesian>(G, i, j, k);
+#pragma unroll
+        for (int dim = 0; dim < ND; ++dim) {
+            double const dS = face_area<ND, kCartesian>(G, dim, i, j, k);
+            double const dSp = face_area<ND, kCartesian>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2));
+            S<ND> a, b;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                a[v] = dtodv * dS * lD[v];
+                b[v] = -(dtodv * dSp * lD[v]);
+            }
+            flux_vjp(pf[0][dim], dim, eos.gamma, a, lpf[0][dim], lg);
+            flux_vjp(pf[1][dim], dim, eos.gamma, b, lpf[1][dim], lg);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) store_planes<ND>(A.face, id, pl, slot(NV, s, d), lpf[s][d]);
+    return lg;
+}
+
+// Slope adjoint of ring-1 cell c along d: with a = (p(c+1) - p(c)) / hp and b = (p(c) - p(c-1)) / hm,
+// ra = lambda_slope * dslope/da / hp goes to p(c+1) (+) and p(c) (-), rb = lambda_slope * dslope/db / hm
+// to p(c) (+) and p(c-1) (-).  `own` receives the face-prim adjoints that flow to p(c) directly.
+template <int ND, int LIM>
+__host__ __device__ __forceinline__ void slope_adjoint(const GridDev& G, const AdjArrays& A, const int* c, int d, S<ND>& own, S<ND>& ra,
+                                                       S<ND>& rb)
+{
+    constexpr int NV = ND + 2;
+    int64_t const pl = G.plane;
+    int64_t const id = flat(G, c[0], c[1], c[2]);
+    S<ND> const l0 = load_planes<ND>(A.face, id, pl, slot(NV, 0, d));
+    S<ND> const l1 = load_planes<ND>(A.face, id, pl, slot(NV, 1, d));
+    S<ND> const p = load_planes<ND>(A.prim, id, pl, 0);
+    S<ND> const pm = load_planes<ND>(A.prim, shifted(G, c, d, -1), pl, 0);
+    S<ND> const pp = load_planes<ND>(A.prim, shifted(G, c, d, +1), pl, 0);
+    double const dl = G.dx[d][c[d]];
+    double const hp = (dl + G.dx[d][c[d] + 1]) / 2;
+    double const hm = (G.dx[d][c[d] - 1] + dl) / 2;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double gR, gL;
+        limiter_grad<LIM>((pp[v] - p[v]) / hp, (p[v] - pm[v]) / hm, gR, gL);
+        double const ls = (dl / 2) * (l1[v] - l0[v]);
+        own[v] = l0[v] + l1[v];
+        ra[v] = ls * gR / hp;
+        rb[v] = ls * gL / hm;
+    }
+}
+
+// Adjoint of the primitive state of cell c (any cell of the array).
+template <int ND, int LIM>
+__host__ __device__ __forceinline__ S<ND> cell_prim_adjoint(const GridDev& G, const AdjArrays& A, const int* c)
+{
+    constexpr int NV = ND + 2;
+    S<ND> lp = zero<ND>();
+    S<ND> own, ra, rb;
+    if (ring1<ND>(G, c)) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            slope_adjoint<ND, LIM>(G, A, c, d, own, ra, rb);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) lp[v] += (own[v] - ra[v]) + rb[v];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        int q[3] = {c[0], c[1], c[2]};
+        q[d] = c[d] - 1;
+        if (ring1<ND>(G, q)) {
+            slope_adjoint<ND, LIM>(G, A, q, d, own, ra, rb);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) lp[v] += ra[v];
+        }
+        q[d] = c[d] + 1;
+        if (ring1<ND>(G, q)) {
+            slope_adjoint<ND, LIM>(G, A, q, d, own, ra, rb);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) lp[v] -= rb[v];
+        }
+    }
+    return lp;
+}
+
+// Cells along dim d whose ghost-filled value is copied from interior index a (a itself first).
+__host__ __device__ __forceinline__ int preimages(const GridDev& G, const BcSpec& bc, int d, int a, int* pre, bool* flip)
+{
+    int m = 0;
+    pre[m] = a;
+    flip[m] = false;
+    ++m;
+    int const ng = G.ngh[d], n = G.n[d], ext = G.nwg[d];
+    for (int g = 0; g < ng; ++g) {
+        bool f = false;
+        if (face_map(bc.kind[2 * d], g, ng, n, ext, f) == a) {
+            pre[m] = g;
+            flip[m] = f;
+            ++m;
+        }
+    }
+    for (int g = ng + n; g < ext; ++g) {
+        bool f = false;
+        if (face_map(bc.kind[2 * d + 1], g, ng, n, ext, f) == a) {
+            pre[m] = g;
+            flip[m] = f;
+            ++m;
+        }
+    }
+    return m;
+}
+
+// ---------------------------------------------------------------- pass 5: gather (interior cells)
+template <int ND, int LIM>
+__host__ __device__ __forceinline__ double cell_gather(const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const AdjArrays& A, int i,
+                                                       int j, int k)
+{
+    constexpr int NV = ND + 2;
+    int64_t const pl = G.plane;
+    int const c[3] = {i, j, k};
+    int64_t const id = flat(G, i, j, k);
+    S<ND> const ln = load_cons<ND>(A.lrho_new, A.lrhou_new, A.lE_new, id, pl);
+    S<ND> out = ln;
+    // godunov_scheme.hpp gravity: U'[1+d] += dt g_d rho, U'[E] += dt g_d m_d
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double const g = gravity(G, i, d);
+        out[0] += A.dt * g * ln[1 + d];
+        out[1 + d] += A.dt * g * ln[ND + 1];
+    }
+    int pre[3][kMaxPre];
+    bool flip[3][kMaxPre];
+    int npre[3] = {1, 1, 1};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        pre[d][0] = c[d];
+        flip[d][0] = false;
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) npre[d] = preimages(G, bc, d, c[d], pre[d], flip[d]);
+    double lg = 0;
+    for (int a = 0; a < npre[0]; ++a)
+        for (int b = 0; b < npre[1]; ++b)
+            for (int e = 0; e < npre[2]; ++e) {
+                int const q[3] = {pre[0][a], pre[1][b], pre[2][e]};
+                S<ND> const lp = cell_prim_adjoint<ND, LIM>(G, A, q);
+                int64_t const qid = flat(G, q[0], q[1], q[2]);
+                S<ND> lu = zero<ND>();
+                prim_vjp(load_cons<ND>(A.rho, A.rhou, A.E, qid, pl), eos.gamma, lp, lu, lg);
+                bool const fl[3] = {flip[0][a], flip[1][b], flip[2][e]};
+                out[0] += lu[0];
+#pragma unroll
+                for (int d = 0; d < ND; ++d) out[1 + d] += fl[d] ? -lu[1 + d] : lu[1 + d];
+                out[ND + 1] += lu[ND + 1];
+            }
+    A.lrho[id] = out[0];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) A.lrhou[id + pl * d] = out[1 + d];
+    A.lE[id] = out[ND + 1];
+    return lg;
+}
+
+}  // namespace hclpp::adj
+
+namespace hclpp {
+
+// adjoint.hip: number of workgroup partials one reverse step writes, and the launcher
+// (passes in the order above, then dgamma[0] += the fixed-order sum of the partials).
+int64_t adjoint_partials(const GridDev& G, int nd);
+void launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const adj::AdjArrays& A,
+                         double* partials, double* dgamma, hipStream_t s);
+
+}  // namespace hclpp

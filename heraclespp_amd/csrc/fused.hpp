@@ -15,6 +15,8 @@ namespace hclpp {
 struct DevClock {
     hclpp_clock pub;
     int pending_exit;
+    int dt_log_cap;   // entries of dt_log
+    double* dt_log;   // when set, step `iter` records its dt at dt_log[iter] (iter < dt_log_cap)
 };
 
 // Per-workgroup partials of the fused step: [0] max CFL rate, [1] min internal
@@ -82,6 +84,9 @@ void launch_state_rate_f32(int nd, int geom, const GridDev& G, const PerfectGasM
 void launch_cast(const double* in, float* out, int64_t n, hipStream_t s);
 void launch_cast(const float* in, double* out, int64_t n, hipStream_t s);
 void launch_clock_start(DevClock* c, const double* inv, hipStream_t s);
+// replace the scheduled dt of the next step (the t_end / max_iter exit rules are re-evaluated)
+void launch_clock_set_dt(DevClock* c, double dt, hipStream_t s);
+void launch_clock_set_log(DevClock* c, double* log, int cap, hipStream_t s);
 void launch_clock_advance(int nd, DevClock* c, const double* partials, int64_t nblocks, const double* ghost_partials, int64_t nghost,
                           hipStream_t s);
 

@@ -41,7 +41,7 @@ __host__ __device__ __forceinline__ int64_t flat(const GridDev& G, int i, int j,
 }
 
 // gravity/gravity.hpp:36-40 (uniform) and :63-70 (point mass)
-__device__ __forceinline__ double gravity(const GridDev& G, int i, int dir)
+__host__ __device__ __forceinline__ double gravity(const GridDev& G, int i, int dir)
 {
     if (G.gravity == kUniformGravity) return G.g[dir];
     return dir == 0 ? G.gpm[i] : 0.;
@@ -49,7 +49,7 @@ __device__ __forceinline__ double gravity(const GridDev& G, int i, int dir)
 
 // mesh/geometry.cpp:32-126: cell volume
 template <int ND, int GEOM>
-__device__ __forceinline__ double cell_volume(const GridDev& G, int i, int j, int k)
+__host__ __device__ __forceinline__ double cell_volume(const GridDev& G, int i, int j, int k)
 {
     if constexpr (GEOM == kCartesian) {
         return G.dx[0][i] * G.dx[1][j] * G.dx[2][k];
@@ -68,7 +68,7 @@ __device__ __forceinline__ double cell_volume(const GridDev& G, int i, int j, in
 
 // mesh/geometry.cpp:32-126: area of the face at the lower side of cell (i,j,k) normal to `dir`
 template <int ND, int GEOM>
-__device__ __forceinline__ double face_area(const GridDev& G, int dir, int i, int j, int k)
+__host__ __device__ __forceinline__ double face_area(const GridDev& G, int dir, int i, int j, int k)
 {
     if constexpr (GEOM == kCartesian) {
         if (dir == 0) return G.dx[1][j] * G.dx[2][k];

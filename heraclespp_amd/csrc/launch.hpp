@@ -45,6 +45,28 @@ struct BcSpec {
     int kind[6];  // per face 2*dim + side
 };
 
+// Source index along one dim of a ghost-filled array: the index of the
+// interior cell whose value a ghost at `idx` receives (periodic wrap,
+// NullGradient clamp, Reflexive mirror with `flip` toggled for the normal
+// momentum); interior indices map to themselves.  ng ghosts, n interior
+// cells, ext = n + 2 ng.
+__host__ __device__ __forceinline__ int face_map(int kind, int idx, int ng, int n, int ext, bool& flip)
+{
+    if (idx < ng) {  // left ghosts
+        if (kind == kPeriodic) return idx + n;
+        if (kind == kNullGradient) return ng;
+        flip = !flip;
+        return (2 * ng) - 1 - idx;
+    }
+    if (idx >= ng + n) {  // right ghosts
+        if (kind == kPeriodic) return idx - n;
+        if (kind == kNullGradient) return ext - ng - 1;
+        flip = !flip;
+        return (2 * (ext - ng)) - 1 - idx;
+    }
+    return idx;
+}
+
 // ---- launchers (ref_kernels.hip): reference-shaped kernels, one per reference kernel label
 void launch_prim_to_cons(int nd, const GridDev& G, const Range3& r, const PerfectGas& eos, const double* rho, const double* u,
                          const double* P, double* rhou, double* E, hipStream_t s);

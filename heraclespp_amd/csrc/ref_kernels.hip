@@ -419,23 +419,6 @@ struct GhostBoxes {
     int64_t start[7];
 };
 
-__device__ __forceinline__ int face_map(int kind, int idx, int ng, int n, int ext, bool& flip)
-{
-    if (idx < ng) {  // left ghosts
-        if (kind == kPeriodic) return idx + n;
-        if (kind == kNullGradient) return ng;
-        flip = !flip;
-        return (2 * ng) - 1 - idx;
-    }
-    if (idx >= ng + n) {  // right ghosts
-        if (kind == kPeriodic) return idx - n;
-        if (kind == kNullGradient) return ext - ng - 1;
-        flip = !flip;
-        return (2 * (ext - ng)) - 1 - idx;
-    }
-    return idx;
-}
-
 template <int ND>
 __global__ __launch_bounds__(256) void k_fill_ghosts(GridDev G, BcSpec bc, GhostBoxes B, double* __restrict__ rho, double* __restrict__ rhou,
                                                      double* __restrict__ E, double* __restrict__ fx, int nfx)

@@ -647,6 +647,21 @@ __global__ __launch_bounds__(64) void k_clock_start(DevClock* c, const double* i
 {
     if (threadIdx.x == 0) schedule_next(c, inv[0]);
 }
+__global__ __launch_bounds__(64) void k_clock_set_dt(DevClock* c, double dt)
+{
+    if (threadIdx.x != 0 || c->pub.done || c->pub.status) return;
+    int pending = 0;
+    if ((c->pub.t + dt) >= c->pub.t_end) pending = 1;
+    if ((c->pub.iter + 1) >= c->pub.max_iter) pending = 1;
+    c->pub.dt = dt;
+    c->pending_exit = pending;
+}
+__global__ __launch_bounds__(64) void k_clock_set_log(DevClock* c, double* log, int cap)
+{
+    if (threadIdx.x != 0) return;
+    c->dt_log = log;
+    c->dt_log_cap = log ? cap : 0;
+}
 
 template <int NV>
 __global__ __launch_bounds__(1024) void k_clock_advance(DevClock* c, const double* __restrict__ partials, int64_t nblocks,
@@ -676,6 +691,7 @@ __global__ __launch_bounds__(1024) void k_clock_advance(DevClock* c, const doubl
         c->pub.status = 3;  // HCLPP_ERR_STATE: the step is discarded
         return;
     }
+    if (c->dt_log && c->pub.iter < c->dt_log_cap) c->dt_log[c->pub.iter] = c->pub.dt;
     c->pub.t += c->pub.dt;
     c->pub.iter += 1;
     c->pub.mass = s[0];
@@ -854,6 +870,11 @@ void launch_cast(const float* in, double* out, int64_t n, hipStream_t s)
 }
 
 void launch_clock_start(DevClock* c, const double* inv, hipStream_t s) { hipLaunchKernelGGL(k_clock_start, dim3(1), dim3(64), 0, s, c, inv); }
+void launch_clock_set_dt(DevClock* c, double dt, hipStream_t s) { hipLaunchKernelGGL(k_clock_set_dt, dim3(1), dim3(64), 0, s, c, dt); }
+void launch_clock_set_log(DevClock* c, double* log, int cap, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_clock_set_log, dim3(1), dim3(64), 0, s, c, log, cap);
+}
 
 void launch_clock_advance(int nd, DevClock* c, const double* partials, int64_t nblocks, const double* ghost_partials, int64_t nghost,
                           hipStream_t s)

@@ -175,6 +175,29 @@ int hclpp_step_f32(hclpp_ctx* ctx, const float* rho, const float* rhou, const fl
 int hclpp_cast_f64_to_f32(hclpp_ctx* ctx, const double* in, float* out, long long n);
 int hclpp_cast_f32_to_f64(hclpp_ctx* ctx, const float* in, double* out, long long n);
 
+/* ---- reverse mode (no counterpart in the reference: the derivative of hclpp_step) ----
+ * The step is differentiated with dt held fixed; the caller replays the forward dt
+ * schedule (recorded with hclpp_clock_record_dt) through hclpp_step_dt.  Cartesian
+ * geometry, HLL or HLLC, any limiter, uniform or point-mass gravity, any boundary
+ * kinds, FP64 storage, nfx = 0, 2 <= ng <= 4; otherwise HCLPP_ERR_UNSUPPORTED. */
+
+/* hclpp_step with the step length dt instead of the clock's CFL dt (t and iter advance
+ * by it; the t_end / max_iter exit rules apply as in hclpp_step). */
+int hclpp_step_dt(hclpp_ctx* ctx, double dt, const double* rho, const double* rhou, const double* E, double* rho_new,
+                  double* rhou_new, double* E_new);
+/* From the next step on, each step stores its dt at log[iter] (device array of cap doubles,
+ * iter < cap); log = NULL stops recording.  hclpp_clock_start resets it. */
+int hclpp_clock_record_dt(hclpp_ctx* ctx, double* log, int cap);
+/* Vector-Jacobian product of one step U -> U' with step length dt:
+ *   (rho, rhou, E)            the step's input, ghosts filled (as hclpp_step takes it);
+ *   (lrho_new, lrhou_new, lE_new)  dL/dU' on the interior (ghost entries are not read);
+ *   (lrho, lrhou, lE)         receive dL/dU on the interior (ghost entries are not written);
+ *   dgamma                    device double; dL/dgamma of this step is added to it.
+ * Deterministic: repeated calls give bit-identical results.  Scratch of
+ * (2 ndim + 1)(ndim + 2) doubles per cell is allocated on the first call. */
+int hclpp_step_adjoint(hclpp_ctx* ctx, double dt, const double* rho, const double* rhou, const double* E, const double* lrho_new,
+                       const double* lrhou_new, const double* lE_new, double* lrho, double* lrhou, double* lE, double* dgamma);
+
 /* ---- instrumentation (no reference counterpart; for benchmarks) ----
  * Between begin and end, each hclpp_step records a HIP event pair around its
  * fused kernel on the context's stream (at most max_steps pairs).  end
