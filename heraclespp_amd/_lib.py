@@ -33,6 +33,7 @@ EXPORTS = [
     "hclpp_hydro_reconstruction", "hclpp_godunov", "hclpp_time_step", "hclpp_min_internal_energy", "hclpp_integrate",
     "hclpp_fill_ghosts", "hclpp_clock_start", "hclpp_clock_read", "hclpp_step", "hclpp_clock_start_f32", "hclpp_step_f32",
     "hclpp_cast_f64_to_f32", "hclpp_cast_f32_to_f64", "hclpp_step_dt", "hclpp_clock_record_dt", "hclpp_step_adjoint",
+    "hclpp_steps_dt", "hclpp_steps_adjoint",
     "hclpp_profile_begin", "hclpp_profile_end",
 ]
 
@@ -125,6 +126,8 @@ def lib() -> ctypes.CDLL:
         "hclpp_step_dt": [vp, ctypes.c_double, dp, dp, dp, dp, dp, dp],
         "hclpp_clock_record_dt": [vp, dp, ip],
         "hclpp_step_adjoint": [vp, ctypes.c_double] + [dp] * 10,
+        "hclpp_steps_dt": [vp, ip, ctypes.POINTER(ctypes.c_double)] + [ctypes.POINTER(ctypes.c_void_p)] * 3,
+        "hclpp_steps_adjoint": [vp, ip, ctypes.POINTER(ctypes.c_double)] + [ctypes.POINTER(ctypes.c_void_p)] * 6 + [dp],
         "hclpp_profile_begin": [vp, ip],
         "hclpp_profile_end": [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_int)],
     }

@@ -32,6 +32,10 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _ptr_array(ts):
+    return (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+
 @dataclass
 class RolloutSpec:
     """Grid and scheme of a rollout (Cartesian geometry, HLL or HLLC, any limiter and boundaries)."""
@@ -94,6 +98,25 @@ class StepEngine:
         check(lib().hclpp_step(self.ctx.handle, _ptr(src[0]), _ptr(src[1]), _ptr(src[2]), _ptr(dst[0]), _ptr(dst[1]),
                                _ptr(dst[2])))
 
+    def steps(self, states, dts):
+        """len(dts) steps, step k from states[k] to states[k + 1] (one C call)."""
+        n = len(dts)
+        if n == 0:
+            return
+        arr = (ctypes.c_double * n)(*dts)
+        cols = [_ptr_array([U[v] for U in states]) for v in range(3)]
+        check(lib().hclpp_steps_dt(self.ctx.handle, n, arr, *cols))
+
+    def vjps(self, states, lams, dgamma, dts):
+        """Reverse steps len(dts)-1 .. 0: lams[k] = VJP of step k (state states[k]) applied to lams[k + 1]."""
+        n = len(dts)
+        if n == 0:
+            return
+        arr = (ctypes.c_double * n)(*dts)
+        cols = [_ptr_array([U[v] for U in states]) for v in range(3)]
+        lcols = [_ptr_array([L[v] for L in lams]) for v in range(3)]
+        check(lib().hclpp_steps_adjoint(self.ctx.handle, n, arr, *cols, *lcols, _ptr(dgamma)))
+
     def vjp(self, U, lam_new, lam, dgamma, dt: float):
         check(lib().hclpp_step_adjoint(self.ctx.handle, float(dt), _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(lam_new[0]),
                                        _ptr(lam_new[1]), _ptr(lam_new[2]), _ptr(lam[0]), _ptr(lam[1]), _ptr(lam[2]),
@@ -131,17 +154,21 @@ def cfl_schedule(spec: RolloutSpec, rho, rhou, E, gamma: float, nsteps: int, cfl
 
 def forward_states(eng: StepEngine, U0, dts, keep):
     """Run the steps dts from state U0; returns {step index: state clone} for the indices in keep, and the final state."""
-    cur = tuple(t.clone() for t in U0)
-    nxt = eng.new_state()
+    N = len(dts)
+    bufs = (tuple(t.clone() for t in U0), eng.new_state())
     kept = {0: tuple(t.clone() for t in U0)} if 0 in keep else {}
-    eng.start(cur)
-    for n, dt in enumerate(dts):
-        eng.step(cur, nxt, dt)
-        cur, nxt = nxt, cur
-        if (n + 1) in keep:
-            kept[n + 1] = tuple(t.clone() for t in cur)
-    eng.check_clock(len(dts))
-    return kept, cur
+    eng.start(bufs[0])
+    marks = sorted(k for k in keep if 0 < k < N) + [N]
+    at, cur = 0, 0
+    for m in marks:
+        n = m - at
+        eng.steps([bufs[(cur + j) % 2] for j in range(n + 1)], dts[at:m])
+        cur = (cur + n) % 2
+        at = m
+        if m in keep:
+            kept[m] = tuple(t.clone() for t in bufs[cur])
+    eng.check_clock(N)
+    return kept, bufs[cur]
 
 
 def binomial_split(steps: int, snaps: int) -> int:
@@ -197,10 +224,13 @@ class _Rollout(torch.autograd.Function):
             state["lam"], state["lam2"] = state["lam2"], state["lam"]
             state["reverse_steps"] += 1
 
+        def batched(seg, s, e):
+            _reverse_segment(eng, seg, s, e, ctx.dts, dgam, state)
+
         N = len(ctx.dts)
         eng.start(ctx.kept[0])  # the clock only gates the recomputation steps here; dt comes from the schedule
         if ctx.schedule == "periodic":
-            _reverse_periodic(eng, ctx.kept, ctx.dts, ctx.every, reverse, state)
+            _reverse_periodic(eng, ctx.kept, ctx.dts, ctx.every, reverse, state, batched)
         else:
             _reverse_binomial(eng, ctx.kept[0], 0, N, ctx.dts, ctx.snapshots, reverse, state)
         ctx.kept = None
@@ -211,19 +241,38 @@ class _Rollout(torch.autograd.Function):
         return out + (dgam.reshape(ctx.gamma_shape).to(ctx.gamma_device), None, None, None, None, None)
 
 
-def _reverse_periodic(eng, kept, dts, every, reverse, state):
+def _reverse_segment(eng, seg, s, e, dts, dgam, state):
+    """Reverse steps e-1 .. s in one call; seg[k - s] is the state before step k.  The adjoints
+    alternate between state["lam"] and state["lam2"] so that adjoint e is the current one."""
+    n = e - s
+    lams = [state["lam"] if (e - k) % 2 == 0 else state["lam2"] for k in range(s, e + 1)]
+    eng.vjps(seg, lams, dgam, dts[s:e])
+    if n % 2:
+        state["lam"], state["lam2"] = state["lam2"], state["lam"]
+    state["reverse_steps"] += n
+
+
+def _reverse_periodic(eng, kept, dts, every, reverse, state, batched=None):
+    """Segments last to first: recompute the segment's states from its checkpoint, then reverse it.
+
+    `batched(states, s, e)` (optional) reverses steps e-1 .. s in one call; otherwise `reverse`
+    is called per step."""
     N = len(dts)
     pool = [eng.new_state() for _ in range(min(every, N))]
     for s in sorted(kept, reverse=True):
         e = min(s + every, N)
-        seg = [kept[s]]
-        for n in range(s, e - 1):
-            dst = pool[n - s]
-            eng.step(seg[-1], dst, dts[n])
-            seg.append(dst)
-            state["forward_steps"] += 1
-        for n in range(e - 1, s - 1, -1):
-            reverse(seg[n - s], n)
+        seg = [kept[s]] + pool[:e - 1 - s]
+        if hasattr(eng, "steps"):
+            eng.steps(seg, dts[s:e - 1])
+        else:
+            for n in range(s, e - 1):
+                eng.step(seg[n - s], seg[n - s + 1], dts[n])
+        state["forward_steps"] += e - 1 - s
+        if batched is not None:
+            batched(seg, s, e)
+        else:
+            for n in range(e - 1, s - 1, -1):
+                reverse(seg[n - s], n)
 
 
 def _reverse_binomial(eng, U, first, last, dts, snaps, reverse, state):
@@ -273,9 +322,16 @@ class DiffEulerProblem:
     """The differentiable-Euler configuration: smooth 2D periodic flow (setups.smooth_euler), a fixed
     CFL dt schedule from the unperturbed state, loss L = 1/2 sum (rho_T - rho_tgt)^2 dV with rho_tgt
     the rollout of a seeded perturbation of rho0.  Inputs of the loss: rho0 (interior) and gamma;
-    u and P are held fixed and the conservative state is formed in torch (differentiable)."""
+    u and P are held fixed and the conservative state is formed in torch (differentiable).
 
-    def __init__(self, n: int, steps: int, seed: int = 0, gamma: float = 1.4, target_amp: float = 0.02):
+    With uniform u and P the scheme carries the density as a pure contact, which HLLC resolves
+    without any dependence on the wave speeds: rho_T, hence this loss, does not depend on gamma.
+    `energy_weight` > 0 adds w/2 sum (E_T - E_tgt)^2 dV, which does (E = P/(gamma-1) + rho u^2/2);
+    its gamma derivative is -P/(gamma-1)^2 sum (E_T - E_tgt) dV, which vanishes for a zero-mean
+    perturbation (energy is conserved), so `target_offset` shifts the perturbation's mean."""
+
+    def __init__(self, n: int, steps: int, seed: int = 0, gamma: float = 1.4, target_amp: float = 0.02,
+                 energy_weight: float = 0.0, target_offset: float = 0.0):
         import numpy as np
 
         from . import setups
@@ -290,15 +346,17 @@ class DiffEulerProblem:
         self.u = dev(s.u[(slice(None),) + it])
         self.P = dev(s.P[it])
         self.gamma = float(gamma)
+        self.energy_weight = float(energy_weight)
         self.dV = 1.0 / (n * n)
         r, m, E = self.cons(self.rho0, torch.tensor(self.gamma, dtype=torch.float64))
         self.dts = cfl_schedule(self.spec, r, m, E, self.gamma, steps, s.cfl)
         xc = (np.arange(n) + 0.5) / n
         X, Y = np.meshgrid(xc, xc)
-        pert = setups.smooth_density(X, Y, seed + 1, modes=8, kmax=4, amp=target_amp) - 1.0
+        pert = setups.smooth_density(X, Y, seed + 1, modes=8, kmax=4, amp=target_amp) - 1.0 + target_offset
         with torch.no_grad():
             rt, mt, et = self.cons(self.rho0 + dev(pert), torch.tensor(self.gamma, dtype=torch.float64))
-            self.target = rollout(rt, mt, et, self.gamma, self.spec, self.dts)[0]
+            out = rollout(rt, mt, et, self.gamma, self.spec, self.dts)
+            self.target, self.target_E = out[0], out[2]
 
     def cons(self, rho0, gamma):
         """E = P / (gamma - 1) + rho |u|^2 / 2, m = rho u."""
@@ -306,9 +364,12 @@ class DiffEulerProblem:
 
     def loss(self, rho0, gamma, every=20, schedule="periodic", snapshots=8):
         r, m, E = self.cons(rho0, gamma)
-        rT = rollout(r, m, E, gamma, self.spec, self.dts, checkpoint_every=every, schedule=schedule,
-                     snapshots=snapshots)[0]
-        return 0.5 * torch.sum((rT - self.target) ** 2) * self.dV
+        rT, _, ET = rollout(r, m, E, gamma, self.spec, self.dts, checkpoint_every=every, schedule=schedule,
+                            snapshots=snapshots)
+        L = 0.5 * torch.sum((rT - self.target) ** 2) * self.dV
+        if self.energy_weight:
+            L = L + 0.5 * self.energy_weight * torch.sum((ET - self.target_E) ** 2) * self.dV
+        return L
 
     def gradient(self, every=20, schedule="periodic", snapshots=8):
         x = self.rho0.clone().requires_grad_(True)
@@ -324,10 +385,14 @@ class DiffEulerProblem:
             for sgn in (1.0, -1.0):
                 gam = self.gamma + sgn * h * vg
                 r, m, E = self.cons(self.rho0 + sgn * h * v, torch.tensor(gam, dtype=torch.float64))
-                out.append(rollout(r, m, E, gam, self.spec, self.dts)[0])
-            Tp, Tm = out
-            return float(torch.sum((Tp - Tm) * (Tp + Tm - 2 * self.target)) * 0.5 * self.dV / (2 * h))
+                out.append(rollout(r, m, E, gam, self.spec, self.dts))
+            (Rp, _, Ep), (Rm, _, Em) = out
+            d = torch.sum((Rp - Rm) * (Rp + Rm - 2 * self.target))
+            if self.energy_weight:
+                d = d + self.energy_weight * torch.sum((Ep - Em) * (Ep + Em - 2 * self.target_E))
+            return float(d * 0.5 * self.dV / (2 * h))
 
 
-def diff_euler_problem(n: int = 1024, steps: int = 400, seed: int = 0) -> DiffEulerProblem:
-    return DiffEulerProblem(n, steps, seed)
+def diff_euler_problem(n: int = 1024, steps: int = 400, seed: int = 0, energy_weight: float = 0.0,
+                       target_offset: float = 0.0) -> DiffEulerProblem:
+    return DiffEulerProblem(n, steps, seed, energy_weight=energy_weight, target_offset=target_offset)

@@ -48,7 +48,7 @@ struct hclpp_ctx {
     GhostShell shell{};
     int64_t fused_nb = 0;
     double* adj_prim = nullptr;     // reverse step: primitives (NV planes)
-    double* adj_face = nullptr;     // reverse step: face slots (2 ND NV planes)
+    double* adj_face = nullptr;     // reverse step: face slots and slope-adjoint shares
     double* adj_partials = nullptr; // reverse step: dgamma partials
     std::vector<hipEvent_t> prof_events;  // pairs (start, stop) per timed fused launch
     int prof_max = 0;
@@ -670,12 +670,35 @@ int hclpp_step_adjoint(hclpp_ctx* c, double dt, const double* rho, const double*
     int const nv = nd + 2;
     if (!c->adj_prim) {
         HCLPP_HIP(hipMalloc(&c->adj_prim, sizeof(double) * c->G.plane * nv));
-        HCLPP_HIP(hipMalloc(&c->adj_face, sizeof(double) * c->G.plane * 2 * nd * nv));
+        HCLPP_HIP(hipMalloc(&c->adj_face, sizeof(double) * c->G.plane * adj::face_planes(nd)));
         HCLPP_HIP(hipMalloc(&c->adj_partials, sizeof(double) * adjoint_partials(c->G, nd)));
     }
     adj::AdjArrays A{rho, rhou, E, c->adj_prim, c->adj_face, lrho_new, lrhou_new, lE_new, lrho, lrhou, lE, dt};
     launch_step_adjoint(nd, c->cfg.riemann, c->cfg.limiter, c->G, c->eos, c->bc, A, c->adj_partials, dgamma, c->stream);
     HCLPP_LAUNCHED("reverse step");
+    return HCLPP_OK;
+}
+
+int hclpp_steps_dt(hclpp_ctx* c, int n, const double* dts, double* const* rho, double* const* rhou, double* const* E)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (n < 0 || (n > 0 && (!dts || !rho || !rhou || !E))) return fail(HCLPP_ERR_ARG, "null argument");
+    for (int k = 0; k < n; ++k) {
+        int const rc = hclpp_step_dt(c, dts[k], rho[k], rhou[k], E[k], rho[k + 1], rhou[k + 1], E[k + 1]);
+        if (rc) return rc;
+    }
+    return HCLPP_OK;
+}
+
+int hclpp_steps_adjoint(hclpp_ctx* c, int n, const double* dts, const double* const* rho, const double* const* rhou, const double* const* E,
+                        double* const* lrho, double* const* lrhou, double* const* lE, double* dgamma)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (n < 0 || (n > 0 && (!dts || !rho || !rhou || !E || !lrho || !lrhou || !lE))) return fail(HCLPP_ERR_ARG, "null argument");
+    for (int k = n - 1; k >= 0; --k) {
+        int const rc = hclpp_step_adjoint(c, dts[k], rho[k], rhou[k], E[k], lrho[k + 1], lrhou[k + 1], lE[k + 1], lrho[k], lrhou[k], lE[k], dgamma);
+        if (rc) return rc;
+    }
     return HCLPP_OK;
 }
 

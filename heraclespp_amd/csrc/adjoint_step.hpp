@@ -13,12 +13,12 @@
 //   flux   (faces touching the interior, one pass per dim)
 //                               lambda_F = dt/dV (lambda'(right) dS - lambda'(left) dS+), Riemann VJP,
 //                               the adjoints of the two face states overwrite their slots
-//   hancock (ring-1 cells)      adjoint of the Hancock predictor: face-state adjoints -> face-prim
-//                               adjoints, overwriting the slots
-//   gather (interior cells)     face-prim adjoints -> cell-prim adjoints through the slope
-//                               limiter (gathering the +-1 neighbours' terms), prim -> cons VJP,
-//                               then the transposed ghost fill (every ghost cell whose value was
-//                               copied from this cell adds its adjoint back, sign-flipped where
+//   hancock (ring-1 cells)      adjoint of the Hancock predictor (face-state adjoints -> face-prim
+//                               adjoints), then of the limited slopes: the cell's own prim adjoint
+//                               and the shares owed to its +-1 neighbours, overwriting the slots
+//   gather (interior cells)     cell-prim adjoint = own part + the neighbours' shares, prim -> cons
+//                               VJP, then the transposed ghost fill (every ghost cell whose value
+//                               was copied from this cell adds its adjoint back, sign-flipped where
 //                               the fill negated the momentum), the identity and gravity terms.
 // Slot (side s, dim d) of a cell holds NV values at planes v + NV * (s + 2 d).
 // Every pass also returns the cell's or face's contribution to dL/dgamma.
@@ -38,7 +38,7 @@ struct AdjArrays {
     const double* rhou;
     const double* E;
     double* prim;       // NV planes
-    double* face;       // 2 ND NV planes (face slots)
+    double* face;       // face_planes(ND) planes: face slots, then the slope-adjoint shares
     const double* lrho_new;  // adjoint of the step output (interior cells read)
     const double* lrhou_new;
     const double* lE_new;
@@ -87,6 +87,11 @@ __host__ __device__ __forceinline__ S<ND> load_cons(const double* rho, const dou
 }
 
 __host__ __device__ __forceinline__ constexpr int slot(int nv, int s, int d) { return nv * (s + 2 * d); }
+// After the Hancock adjoint a ring-1 cell's planes hold: [0, nv) its own prim adjoint, then per dim d
+// the share of cell c + e_d (share_up) and of cell c - e_d (share_down): nv (2 nd + 1) planes in all.
+__host__ __device__ __forceinline__ constexpr int share_up(int nv, int d) { return nv * (1 + d); }
+__host__ __device__ __forceinline__ constexpr int share_down(int nv, int nd, int d) { return nv * (1 + nd + d); }
+constexpr int face_planes(int nd) { return (nd + 2) * (2 * nd + 1); }
 
 __host__ __device__ __forceinline__ int64_t shifted(const GridDev& G, const int* c, int d, int by)
 {
@@ -225,6 +230,34 @@ __host__ __device__ __forceinline__ bool slot_live(const GridDev& G, const int* 
     return inside(G, d, c[d]) || inside(G, d, c[d] + (s ? 1 : -1));
 }
 
+// Slope adjoint of ring-1 cell c along d (limited_linear_reconstruction.hpp:46-64 reversed): with
+// a = (p(c+1) - p(c)) / hp and b = (p(c) - p(c-1)) / hm, ra = lambda_slope * dslope/da / hp goes to
+// p(c+1) (+) and p(c) (-), rb = lambda_slope * dslope/db / hm to p(c) (+) and p(c-1) (-); `own`
+// receives the face-prim adjoints l0, l1 that flow to p(c) directly.
+template <int ND, int LIM>
+__host__ __device__ __forceinline__ void slope_adjoint(const GridDev& G, const double* prim, const int* c, int d, const S<ND>& l0,
+                                                       const S<ND>& l1, S<ND>& own, S<ND>& ra, S<ND>& rb)
+{
+    constexpr int NV = ND + 2;
+    int64_t const pl = G.plane;
+    int64_t const id = flat(G, c[0], c[1], c[2]);
+    S<ND> const p = load_planes<ND>(prim, id, pl, 0);
+    S<ND> const pm = load_planes<ND>(prim, shifted(G, c, d, -1), pl, 0);
+    S<ND> const pp = load_planes<ND>(prim, shifted(G, c, d, +1), pl, 0);
+    double const dl = G.dx[d][c[d]];
+    double const hp = (dl + G.dx[d][c[d] + 1]) / 2;
+    double const hm = (G.dx[d][c[d] - 1] + dl) / 2;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double gR, gL;
+        limiter_grad<LIM>((pp[v] - p[v]) / hp, (p[v] - pm[v]) / hm, gR, gL);
+        double const ls = (dl / 2) * (l1[v] - l0[v]);
+        own[v] = l0[v] + l1[v];
+        ra[v] = ls * gR / hp;
+        rb[v] = ls * gL / hm;
+    }
+}
+
 // ---------------------------------------------------------------- pass 4: Hancock adjoint (ring-1 cells)
 template <int ND, int LIM>
 __host__ __device__ __forceinline__ double cell_hancock(const GridDev& G, const PerfectGas& eos, const AdjArrays& A, int i, int j, int k)
@@ -286,69 +319,42 @@ __host__ __device__ __forceinline__ double cell_hancock(const GridDev& G, const 
             flux_vjp(pf[1][dim], dim, eos.gamma, b, lpf[1][dim], lg);
         }
     }
+    // slope adjoint: the face-prim adjoints become the cell's own prim adjoint and the shares of
+    // its +-1 neighbours along each dim, stored in place of the face slots
+    S<ND> self = zero<ND>();
 #pragma unroll
-    for (int d = 0; d < ND; ++d)
+    for (int d = 0; d < ND; ++d) {
+        S<ND> own = zero<ND>(), ra = zero<ND>(), rb = zero<ND>();
+        if (any) slope_adjoint<ND, LIM>(G, A.prim, c, d, lpf[0][d], lpf[1][d], own, ra, rb);
 #pragma unroll
-        for (int s = 0; s < 2; ++s) store_planes<ND>(A.face, id, pl, slot(NV, s, d), lpf[s][d]);
+        for (int v = 0; v < NV; ++v) self[v] += (own[v] - ra[v]) + rb[v];
+        store_planes<ND>(A.face, id, pl, share_up(NV, d), ra);
+        store_planes<ND>(A.face, id, pl, share_down(NV, ND, d), rb);
+    }
+    store_planes<ND>(A.face, id, pl, 0, self);
     return lg;
 }
 
-// Slope adjoint of ring-1 cell c along d: with a = (p(c+1) - p(c)) / hp and b = (p(c) - p(c-1)) / hm,
-// ra = lambda_slope * dslope/da / hp goes to p(c+1) (+) and p(c) (-), rb = lambda_slope * dslope/db / hm
-// to p(c) (+) and p(c-1) (-).  `own` receives the face-prim adjoints that flow to p(c) directly.
-template <int ND, int LIM>
-__host__ __device__ __forceinline__ void slope_adjoint(const GridDev& G, const AdjArrays& A, const int* c, int d, S<ND>& own, S<ND>& ra,
-                                                       S<ND>& rb)
-{
-    constexpr int NV = ND + 2;
-    int64_t const pl = G.plane;
-    int64_t const id = flat(G, c[0], c[1], c[2]);
-    S<ND> const l0 = load_planes<ND>(A.face, id, pl, slot(NV, 0, d));
-    S<ND> const l1 = load_planes<ND>(A.face, id, pl, slot(NV, 1, d));
-    S<ND> const p = load_planes<ND>(A.prim, id, pl, 0);
-    S<ND> const pm = load_planes<ND>(A.prim, shifted(G, c, d, -1), pl, 0);
-    S<ND> const pp = load_planes<ND>(A.prim, shifted(G, c, d, +1), pl, 0);
-    double const dl = G.dx[d][c[d]];
-    double const hp = (dl + G.dx[d][c[d] + 1]) / 2;
-    double const hm = (G.dx[d][c[d] - 1] + dl) / 2;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        double gR, gL;
-        limiter_grad<LIM>((pp[v] - p[v]) / hp, (p[v] - pm[v]) / hm, gR, gL);
-        double const ls = (dl / 2) * (l1[v] - l0[v]);
-        own[v] = l0[v] + l1[v];
-        ra[v] = ls * gR / hp;
-        rb[v] = ls * gL / hm;
-    }
-}
-
-// Adjoint of the primitive state of cell c (any cell of the array).
-template <int ND, int LIM>
+// Adjoint of the primitive state of cell c (any cell of the array): its own part plus the
+// shares of its +-1 neighbours along each dim (zero outside the ring-1 cells).
+template <int ND>
 __host__ __device__ __forceinline__ S<ND> cell_prim_adjoint(const GridDev& G, const AdjArrays& A, const int* c)
 {
     constexpr int NV = ND + 2;
-    S<ND> lp = zero<ND>();
-    S<ND> own, ra, rb;
-    if (ring1<ND>(G, c)) {
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            slope_adjoint<ND, LIM>(G, A, c, d, own, ra, rb);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) lp[v] += (own[v] - ra[v]) + rb[v];
-        }
-    }
+    int64_t const pl = G.plane;
+    S<ND> lp = ring1<ND>(G, c) ? load_planes<ND>(A.face, flat(G, c[0], c[1], c[2]), pl, 0) : zero<ND>();
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         int q[3] = {c[0], c[1], c[2]};
         q[d] = c[d] - 1;
         if (ring1<ND>(G, q)) {
-            slope_adjoint<ND, LIM>(G, A, q, d, own, ra, rb);
+            S<ND> const ra = load_planes<ND>(A.face, flat(G, q[0], q[1], q[2]), pl, share_up(NV, d));
 #pragma unroll
             for (int v = 0; v < NV; ++v) lp[v] += ra[v];
         }
         q[d] = c[d] + 1;
         if (ring1<ND>(G, q)) {
-            slope_adjoint<ND, LIM>(G, A, q, d, own, ra, rb);
+            S<ND> const rb = load_planes<ND>(A.face, flat(G, q[0], q[1], q[2]), pl, share_down(NV, ND, d));
 #pragma unroll
             for (int v = 0; v < NV; ++v) lp[v] -= rb[v];
         }
@@ -416,7 +422,7 @@ __host__ __device__ __forceinline__ double cell_gather(const GridDev& G, const P
         for (int b = 0; b < npre[1]; ++b)
             for (int e = 0; e < npre[2]; ++e) {
                 int const q[3] = {pre[0][a], pre[1][b], pre[2][e]};
-                S<ND> const lp = cell_prim_adjoint<ND, LIM>(G, A, q);
+                S<ND> const lp = cell_prim_adjoint<ND>(G, A, q);
                 int64_t const qid = flat(G, q[0], q[1], q[2]);
                 S<ND> lu = zero<ND>();
                 prim_vjp(load_cons<ND>(A.rho, A.rhou, A.E, qid, pl), eos.gamma, lp, lu, lg);

@@ -194,9 +194,19 @@ int hclpp_clock_record_dt(hclpp_ctx* ctx, double* log, int cap);
  *   (lrho, lrhou, lE)         receive dL/dU on the interior (ghost entries are not written);
  *   dgamma                    device double; dL/dgamma of this step is added to it.
  * Deterministic: repeated calls give bit-identical results.  Scratch of
- * (2 ndim + 1)(ndim + 2) doubles per cell is allocated on the first call. */
+ * (2 ndim + 2)(ndim + 2) doubles per cell is allocated on the first call. */
 int hclpp_step_adjoint(hclpp_ctx* ctx, double dt, const double* rho, const double* rhou, const double* E, const double* lrho_new,
                        const double* lrhou_new, const double* lE_new, double* lrho, double* lrhou, double* lE, double* dgamma);
+
+/* Batched forms (one call per run of steps, to keep host overhead off the critical path):
+ * hclpp_steps_dt runs n steps, step k reading state k and writing state k + 1, with the
+ * state pointers given as arrays of n + 1 entries per variable (entries k and k + 1 must
+ * differ; ping-pong buffers are fine).  hclpp_steps_adjoint runs the n reverse steps from
+ * last to first: reverse step k uses state k (arrays of n entries) and maps adjoint k + 1
+ * to adjoint k (arrays of n + 1 entries); every step adds to *dgamma. */
+int hclpp_steps_dt(hclpp_ctx* ctx, int n, const double* dts, double* const* rho, double* const* rhou, double* const* E);
+int hclpp_steps_adjoint(hclpp_ctx* ctx, int n, const double* dts, const double* const* rho, const double* const* rhou,
+                        const double* const* E, double* const* lrho, double* const* lrhou, double* const* lE, double* dgamma);
 
 /* ---- instrumentation (no reference counterpart; for benchmarks) ----
  * Between begin and end, each hclpp_step records a HIP event pair around its

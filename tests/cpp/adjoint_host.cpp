@@ -108,7 +108,7 @@ int adjh_step(int nd, int rs, int lim, const int* n, int ng, const double* const
     PerfectGas eos{gamma, gamma - 1, 1.0};
     BcSpec bc{};
     for (int f = 0; f < 6; ++f) bc.kind[f] = f < 2 * nd ? bcs[f] : 0;
-    std::vector<double> prim((size_t)G.plane * (nd + 2)), face((size_t)G.plane * 2 * nd * (nd + 2));
+    std::vector<double> prim((size_t)G.plane * (nd + 2)), face((size_t)G.plane * adj::face_planes(nd));
     adj::AdjArrays A{rho, rhou, E, prim.data(), face.data(), lrho_new, lrhou_new, lE_new, lrho, lrhou, lE, dt};
     double lg;
     switch (nd) {

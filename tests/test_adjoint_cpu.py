@@ -123,14 +123,14 @@ class _CountingEngine:
         import torch
 
         self.torch = torch
-        self.steps = 0
+        self.count = 0
 
     def new_state(self):
         return (self.torch.zeros(1, dtype=self.torch.float64),)
 
     def step(self, src, dst, dt):
         dst[0].copy_(src[0] + 1)
-        self.steps += 1
+        self.count += 1
 
 
 def _reverse_order(run):
@@ -172,3 +172,37 @@ def test_binomial_schedule_reverses_every_step_within_the_revolve_bound(N, snaps
         assert fwd <= t * N  # at most t sweeps of recomputation
     else:
         assert fwd == N * (N - 1) // 2
+
+
+class _BatchedEngine(_CountingEngine):
+    """Batched forms: steps() over a list of states, and a reverse map lam_k = 2 lam_{k+1} + state_k."""
+
+    def steps(self, states, dts):
+        for k in range(len(dts)):
+            self.step(states[k], states[k + 1], dts[k])
+
+    def vjps(self, states, lams, dgam, dts):
+        for k in range(len(dts) - 1, -1, -1):
+            lams[k][0].copy_(2 * lams[k + 1][0] + states[k][0])
+            dgam += 1
+
+
+@pytest.mark.parametrize("N,every", [(1, 20), (7, 3), (40, 20), (41, 20), (23, 23), (9, 2)])
+def test_batched_segments_chain_the_adjoint_buffers(N, every):
+    """The batched reverse pass gives the same result as the per-step recursion lam_k = 2 lam_{k+1} + k."""
+    from heraclespp_amd.adjoint import _reverse_periodic, _reverse_segment
+
+    torch = pytest.importorskip("torch")
+    eng = _BatchedEngine()
+    kept = {s: (torch.full((1,), float(s), dtype=torch.float64),) for s in range(0, N, every)}
+    dgam = torch.zeros(1, dtype=torch.float64)
+    state = {"lam": (torch.full((1,), 1.0, dtype=torch.float64),), "lam2": (torch.zeros(1, dtype=torch.float64),),
+             "reverse_steps": 0, "forward_steps": 0}
+    _reverse_periodic(eng, kept, [0.1] * N, every, None, state,
+                      lambda seg, s, e: _reverse_segment(eng, seg, s, e, [0.1] * N, dgam, state))
+    want = 1.0
+    for k in range(N - 1, -1, -1):
+        want = 2 * want + k
+    assert float(state["lam"][0]) == want
+    assert state["reverse_steps"] == N and float(dgam) == N
+    assert state["forward_steps"] == N - len(kept)
