@@ -21,7 +21,7 @@ HCLPP_ERR_UNSUPPORTED = 4
 GEOM = {"Cartesian": 0, "Spherical": 1}
 LIMITER = {"Constant": 0, "VanLeer": 1, "Minmod": 2, "VanAlbada": 3}
 RIEMANN = {"HLL": 0, "HLLC": 1, "Low Mach": 2}
-GRAVITY = {"Uniform": 0, "Point_mass": 1}
+GRAVITY = {"Uniform": 0, "Point_mass": 1, "Internal_mass": 2}
 BC = {"Periodic": 0, "NullGradient": 1, "Reflexive": 2}
 PRECISION = {"fp64": 0, "fp32": 1}
 
@@ -33,7 +33,7 @@ EXPORTS = [
     "hclpp_hydro_reconstruction", "hclpp_godunov", "hclpp_time_step", "hclpp_min_internal_energy", "hclpp_integrate",
     "hclpp_fill_ghosts", "hclpp_clock_start", "hclpp_clock_read", "hclpp_step", "hclpp_clock_start_f32", "hclpp_step_f32",
     "hclpp_cast_f64_to_f32", "hclpp_cast_f32_to_f64", "hclpp_step_dt", "hclpp_clock_record_dt", "hclpp_step_adjoint",
-    "hclpp_steps_dt", "hclpp_steps_adjoint",
+    "hclpp_steps_dt", "hclpp_steps_adjoint", "hclpp_pressure_fix", "hclpp_internal_mass_gravity", "hclpp_gravity_table",
     "hclpp_profile_begin", "hclpp_profile_end",
 ]
 
@@ -128,6 +128,9 @@ def lib() -> ctypes.CDLL:
         "hclpp_step_adjoint": [vp, ctypes.c_double] + [dp] * 10,
         "hclpp_steps_dt": [vp, ip, ctypes.POINTER(ctypes.c_double)] + [ctypes.POINTER(ctypes.c_void_p)] * 3,
         "hclpp_steps_adjoint": [vp, ip, ctypes.POINTER(ctypes.c_double)] + [ctypes.POINTER(ctypes.c_void_p)] * 6 + [dp],
+        "hclpp_pressure_fix": [vp, rp, d, d] + [dp] * 9,
+        "hclpp_internal_mass_gravity": [vp, dp],
+        "hclpp_gravity_table": [vp, ctypes.POINTER(ctypes.c_double)],
         "hclpp_profile_begin": [vp, ip],
         "hclpp_profile_end": [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_int)],
     }

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -47,6 +48,8 @@ struct hclpp_ctx {
     int64_t fx_old_cap = 0;
     GhostShell shell{};
     int64_t fused_nb = 0;
+    double* gtab = nullptr;         // radial gravity table (inside tables; rebuilt for internal-mass gravity)
+    double* shells = nullptr;       // internal-mass gravity: per-layer sums of rho dV and dV
     double* adj_prim = nullptr;     // reverse step: primitives (NV planes)
     double* adj_face = nullptr;     // reverse step: face slots and slope-adjoint shares
     double* adj_partials = nullptr; // reverse step: dgamma partials
@@ -113,6 +116,13 @@ int ensure_partials(hclpp_ctx* c, int64_t n)
     return HCLPP_OK;
 }
 
+// The clock's done/status pair as a kernel gate (the two ints are adjacent in hclpp_clock).
+const DevFlag* clock_gate(const hclpp_ctx* c)
+{
+    static_assert(offsetof(hclpp_clock, status) == offsetof(hclpp_clock, done) + sizeof(int), "done, status adjacent");
+    return reinterpret_cast<const DevFlag*>(&c->clock->pub.done);
+}
+
 int read_scalar(hclpp_ctx* c, double* out)
 {
     HCLPP_HIP(hipMemcpyAsync(c->host_scalars, c->scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -167,7 +177,8 @@ int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out)
     if (cfg->geom != HCLPP_GEOM_CARTESIAN && cfg->geom != HCLPP_GEOM_SPHERICAL) return fail(HCLPP_ERR_ARG, "unknown geometry");
     if (cfg->limiter < 0 || cfg->limiter > 3) return fail(HCLPP_ERR_ARG, "unknown limiter");
     if (cfg->riemann < 0 || cfg->riemann > 2) return fail(HCLPP_ERR_ARG, "unknown Riemann solver");
-    if (cfg->gravity != HCLPP_GRAV_UNIFORM && cfg->gravity != HCLPP_GRAV_POINT_MASS) return fail(HCLPP_ERR_ARG, "unknown gravity");
+    if (cfg->gravity != HCLPP_GRAV_UNIFORM && cfg->gravity != HCLPP_GRAV_POINT_MASS && cfg->gravity != HCLPP_GRAV_INTERNAL_MASS)
+        return fail(HCLPP_ERR_ARG, "unknown gravity");
     if (cfg->nfx < 0) return fail(HCLPP_ERR_ARG, "nfx < 0");
     if (cfg->ng < 0) return fail(HCLPP_ERR_ARG, "ng < 0");
     if (cfg->precision != HCLPP_FP64 && cfg->precision != HCLPP_FP32_STORAGE) return fail(HCLPP_ERR_ARG, "unknown precision");
@@ -195,7 +206,8 @@ int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out)
     }
     G.plane = (int64_t)G.nwg[0] * G.nwg[1] * G.nwg[2];
     for (int d = 0; d < 3; ++d) G.g[d] = cfg->g[d];
-    G.gravity = cfg->gravity;
+    // internal-mass gravity is a radial table like the point-mass one, rebuilt from rho every step
+    G.gravity = cfg->gravity == HCLPP_GRAV_UNIFORM ? kUniformGravity : kPointMassGravity;
     c->eos.gamma = cfg->gamma;
     c->eos.gamma_m1 = cfg->gamma - 1;  // perfect_gas.cpp:13
     c->eos.mmw = cfg->mmw;
@@ -256,6 +268,11 @@ int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out)
     G.cot_mid = c->tables + off_cot;
     G.sin_center = c->tables + off_sinc;
     G.gpm = c->tables + off_gpm;
+    c->gtab = c->tables + off_gpm;
+    if (cfg->gravity == HCLPP_GRAV_INTERNAL_MASS) {
+        e = hipMalloc(&c->shells, sizeof(double) * 2 * (G.n[0] + G.ngh[0]));
+        if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(shells)"));
+    }
 
     e = hipMalloc(&c->scalars, sizeof(double) * 16);
     if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(scalars)"));
@@ -301,6 +318,7 @@ int hclpp_ctx_destroy(hclpp_ctx* c)
     if (c->clock) (void)hipFree(c->clock);
     if (c->host_clock) (void)hipHostFree(c->host_clock);
     if (c->fx_old) (void)hipFree(c->fx_old);
+    if (c->shells) (void)hipFree(c->shells);
     if (c->adj_prim) (void)hipFree(c->adj_prim);
     if (c->adj_face) (void)hipFree(c->adj_face);
     if (c->adj_partials) (void)hipFree(c->adj_partials);
@@ -467,6 +485,37 @@ int hclpp_integrate(hclpp_ctx* c, const hclpp_range* r, const double* var, doubl
     return read_scalar(c, out);
 }
 
+int hclpp_pressure_fix(hclpp_ctx* c, const hclpp_range* r, double dt, double eps, const double* rho, const double* rhou, const double* E,
+                       const double* rho_rec, const double* rhou_rec, const double* E_rec, const double* rho_new, const double* rhou_new,
+                       double* E_new)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    int rc = check_range(c, r, 1);
+    if (rc) return rc;
+    if (!rho || !rhou || !E || !rho_rec || !rhou_rec || !E_rec || !rho_new || !rhou_new || !E_new) return fail(HCLPP_ERR_ARG, "null array");
+    launch_pressure_fix(c->cfg.ndim, c->cfg.geom, c->G, to_range(r), c->eos, dt, eps, rho, rhou, E, rho_rec, rhou_rec, E_rec, rho_new,
+                        rhou_new, E_new, c->stream);
+    HCLPP_LAUNCHED("pressure_fix");
+    return HCLPP_OK;
+}
+
+int hclpp_internal_mass_gravity(hclpp_ctx* c, const double* rho)
+{
+    if (!c || !rho) return fail(HCLPP_ERR_ARG, "null argument");
+    if (c->cfg.gravity != HCLPP_GRAV_INTERNAL_MASS) return fail(HCLPP_ERR_ARG, "the context was not created with internal-mass gravity");
+    launch_internal_mass_gravity(c->cfg.ndim, c->cfg.geom, c->G, c->cfg.central_mass, rho, c->shells, c->gtab, nullptr, c->stream);
+    HCLPP_LAUNCHED("internal_mass_gravity");
+    return HCLPP_OK;
+}
+
+int hclpp_gravity_table(hclpp_ctx* c, double* host_out)
+{
+    if (!c || !host_out) return fail(HCLPP_ERR_ARG, "null argument");
+    HCLPP_HIP(hipMemcpyAsync(host_out, c->gtab, sizeof(double) * c->G.nwg[0], hipMemcpyDeviceToHost, c->stream));
+    HCLPP_HIP(hipStreamSynchronize(c->stream));
+    return HCLPP_OK;
+}
+
 int hclpp_fill_ghosts(hclpp_ctx* c, double* rho, double* rhou, double* E, double* fx)
 {
     if (!c) return fail(HCLPP_ERR_ARG, "null context");
@@ -528,6 +577,10 @@ int hclpp_step_f32(hclpp_ctx* c, const float* rho, const float* rhou, const floa
     A.ghost_partials = c->ghost_partials;
     A.clock = c->clock;
     int const nd = c->cfg.ndim;
+    if (c->cfg.gravity == HCLPP_GRAV_INTERNAL_MASS) {
+        launch_internal_mass_gravity(nd, c->cfg.geom, c->G, c->cfg.central_mass, rho, c->shells, c->gtab, clock_gate(c), c->stream);
+        HCLPP_LAUNCHED("internal_mass_gravity");
+    }
     bool const timed = c->prof_count < c->prof_max;
     if (timed) HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count], c->stream));
     launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->eos32, A, c->stream);
@@ -617,6 +670,10 @@ int hclpp_step(hclpp_ctx* c, const double* rho, const double* rhou, const double
     A.ghost_partials = c->ghost_partials;
     A.clock = c->clock;
     int const nd = c->cfg.ndim;
+    if (c->cfg.gravity == HCLPP_GRAV_INTERNAL_MASS) {  // main.cpp:309, :528: gravity from the state the step reads
+        launch_internal_mass_gravity(nd, c->cfg.geom, c->G, c->cfg.central_mass, rho, c->shells, c->gtab, clock_gate(c), c->stream);
+        HCLPP_LAUNCHED("internal_mass_gravity");
+    }
     bool const timed = c->prof_count < c->prof_max;
     if (timed) HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count], c->stream));
     launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->eos, A, c->stream);
@@ -661,6 +718,8 @@ int hclpp_step_adjoint(hclpp_ctx* c, double dt, const double* rho, const double*
     if (c->cfg.riemann != HCLPP_RS_HLL && c->cfg.riemann != HCLPP_RS_HLLC)
         return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step supports the HLL and HLLC solvers only");
     if (c->cfg.nfx > 0) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step has no passive scalars");
+    if (c->cfg.gravity == HCLPP_GRAV_INTERNAL_MASS)
+        return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step does not differentiate the state-dependent internal-mass gravity");
     if (c->cfg.ng < 2 || c->cfg.ng > adj::kMaxGhosts) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step needs 2 <= ng <= 4");
     if (!std::isfinite(dt) || !(dt > 0)) return fail(HCLPP_ERR_ARG, "dt must be finite and > 0");
     if (!rho || !rhou || !E || !lrho_new || !lrhou_new || !lE_new || !lrho || !lrhou || !lE || !dgamma)

@@ -67,6 +67,15 @@ __host__ __device__ __forceinline__ int face_map(int kind, int idx, int ng, int 
     return idx;
 }
 
+// Optional gate read by a kernel before it does any work: a kernel enqueued behind the
+// device clock skips itself once the clock has finished or failed (the step it belongs to
+// is then a no-op too).  A null pointer means "always run".
+struct DevFlag {
+    int done;
+    int status;
+    __device__ __forceinline__ bool go() const { return !done && !status; }
+};
+
 // ---- launchers (ref_kernels.hip): reference-shaped kernels, one per reference kernel label
 void launch_prim_to_cons(int nd, const GridDev& G, const Range3& r, const PerfectGas& eos, const double* rho, const double* u,
                          const double* P, double* rhou, double* E, hipStream_t s);
@@ -92,6 +101,14 @@ void launch_min_internal_energy(int nd, const GridDev& G, const Range3& r, const
 void launch_integrate(const GridDev& G, int nd, int geom, const Range3& r, const double* var, double* partials, double* out, hipStream_t s);
 // Fold n partials (max) into out[0] in a fixed order.
 void launch_fold_max(const double* partials, int64_t n, double* out, hipStream_t s);
+// hydro/pressure_fix.hpp:19-172 (E_new updated in place)
+void launch_pressure_fix(int nd, int geom, const GridDev& G, const Range3& r, const PerfectGas& eos, double dt, double eps, const double* rho,
+                         const double* rhou, const double* E, const double* rho_rec, const double* rhou_rec, const double* E_rec,
+                         const double* rho_new, const double* rhou_new, double* E_new, hipStream_t s);
+// gravity/gravity.cpp:59-128: radial gravity table gtab (nwg[0]) from rho; shells holds 2 (n0 + ng) doubles
+template <class S>
+void launch_internal_mass_gravity(int nd, int geom, const GridDev& G, double central_mass, const S* rho, double* shells, double* gtab,
+                                  const DevFlag* run, hipStream_t s);
 // Ghost fill on conservative arrays (boundary_distribute.hpp:42-76 semantics, single rank)
 void launch_fill_ghosts(int nd, const GridDev& G, const BcSpec& bc, double* rho, double* rhou, double* E, double* fx, int nfx, hipStream_t s);
 

@@ -599,4 +599,171 @@ void launch_fill_ghosts(int nd, const GridDev& G, const BcSpec& bc, double* rho,
     HCLPP_ND_SWITCH(nd, hipLaunchKernelGGL(k_fill_ghosts<ND>, dim3(blocks), dim3(256), 0, s, G, bc, B, rho, rhou, E, fx, nfx));
 }
 
+// ------------------------------------------------------------ pressure fix
+// hydro/pressure_fix.hpp:19-172.  For a cell whose internal energy is below eps times its
+// kinetic energy, E_new is rebuilt from the internal-energy equation de = dt (-P div u -
+// div(u e)) (face velocities from the predicted face states, upwinded cell internal
+// energies), blended with the Godunov change by alpha = (e / (eps ek))^2 (0 below eps / 10).
+// The double loop over (idim, ipos) and the face areas of direction ipos are the reference's.
+template <int ND, int GEOM>
+__global__ __launch_bounds__(256) void k_pressure_fix(GridDev G, Range3 r, PerfectGas eos, double dt, double eps,
+                                                      const double* __restrict__ rho, const double* __restrict__ rhou,
+                                                      const double* __restrict__ E, const double* __restrict__ rho_rec,
+                                                      const double* __restrict__ rhou_rec, const double* __restrict__ E_rec,
+                                                      const double* __restrict__ rho_new, const double* __restrict__ rhou_new,
+                                                      double* __restrict__ E_new)
+{
+    int i, j, k;
+    if (!range_cell(r, i, j, k)) return;
+    int64_t const pl = G.plane;
+    int64_t const id = flat(G, i, j, k);
+    auto cell = [&](int64_t c) {
+        State<ND> q;
+        q[0] = rho[c];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) q[1 + d] = rhou[c + pl * d];
+        q[ND + 1] = E[c];
+        return q;
+    };
+    auto face = [&](int64_t c, int s, int idim) {
+        State<ND> q;
+        int64_t const off = c + pl * (s + 2 * idim);
+        q[0] = rho_rec[off];
+#pragma unroll
+        for (int v = 0; v < ND; ++v) q[1 + v] = rhou_rec[off + pl * 2 * ND * v];
+        q[ND + 1] = E_rec[off];
+        return q;
+    };
+    State<ND> const cons = cell(id);
+    State<ND> cons_new;
+    cons_new[0] = rho_new[id];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) cons_new[1 + d] = rhou_new[id + pl * d];
+    cons_new[ND + 1] = E_new[id];
+    State<ND> const prim = cons_to_prim(cons, eos);
+    double const ekin = kinetic_cons(cons);
+    double const evol = E[id] - ekin;
+    double const evol_new = cons_new[ND + 1] - kinetic_cons(cons_new);
+    double const de_god = evol_new - evol;
+    if (!(evol < eps * ekin)) return;
+    double const eps2 = eps / 10;
+    double divU = 0, divUE = 0, einL = 0, einR = 0;
+    double alpha = (evol * evol) / (ekin * ekin * eps * eps);
+    if (evol < eps2 * ekin) alpha = 0;
+    double out = E[id];
+#pragma unroll
+    for (int idim = 0; idim < ND; ++idim) {
+        int const di = idim == 0, dj = idim == 1, dk = idim == 2;
+        int64_t const idm = flat(G, i - di, j - dj, k - dk);
+        int64_t const idp = flat(G, i + di, j + dj, k + dk);
+        State<ND> const primL = cons_to_prim(face(id, 0, idim), eos);
+        State<ND> const primR = cons_to_prim(face(id, 1, idim), eos);
+        State<ND> const prim_mR = cons_to_prim(face(idm, 1, idim), eos);
+        State<ND> const prim_pL = cons_to_prim(face(idp, 0, idim), eos);
+        State<ND> const cons_p = cell(idp);
+        State<ND> const cons_m = cell(idm);
+        double const dv = cell_volume<ND, GEOM>(G, i, j, k);
+#pragma unroll
+        for (int ipos = 0; ipos < ND; ++ipos) {
+            double const us = (prim_mR[1 + ipos] + primL[1 + ipos]) / 2;
+            double const us_p = (primR[1 + ipos] + prim_pL[1 + ipos]) / 2;
+            double const dsp = face_area<ND, GEOM>(G, ipos, i + di, j + dj, k + dk);
+            double const ds0 = face_area<ND, GEOM>(G, ipos, i, j, k);
+            divU += ((us_p * dsp) - (us * ds0)) / dv;
+            einR = (us_p > 0) ? E[id] - kinetic_cons(cons) : E[idp] - kinetic_cons(cons_p);
+            einL = (us < 0) ? E[id] - kinetic_cons(cons) : E[idm] - kinetic_cons(cons_m);
+            divUE += ((us_p * einR * dsp) - (us * einL * ds0)) / dv;
+            double const de_pf = dt * ((-prim[ND + 1] * divU) - divUE);
+            double sum = 0;
+#pragma unroll
+            for (int idr = 0; idr < ND; ++idr) sum += cons[1 + idr] * cons[1 + idr] / cons[0];
+            double const de_tot = (alpha * de_god) + ((1 - alpha) * de_pf);
+            out = evol + de_tot + (0.5 * sum);
+        }
+    }
+    E_new[id] = out;
+}
+
+void launch_pressure_fix(int nd, int geom, const GridDev& G, const Range3& r, const PerfectGas& eos, double dt, double eps, const double* rho,
+                         const double* rhou, const double* E, const double* rho_rec, const double* rhou_rec, const double* E_rec,
+                         const double* rho_new, const double* rhou_new, double* E_new, hipStream_t s)
+{
+    if (empty(r)) return;
+    HCLPP_GEOM_SWITCH(geom, HCLPP_ND_SWITCH(nd, hipLaunchKernelGGL((k_pressure_fix<ND, GEOM>), range_grid(r), dim3(kBX, kBY), 0, s, G, r, eos,
+                                                                   dt, eps, rho, rhou, E, rho_rec, rhou_rec, E_rec, rho_new, rhou_new,
+                                                                   E_new)));
+}
+
+// ------------------------------------------------------------ internal-mass gravity
+// gravity/gravity.cpp:59-128 on one rank along dim 0.  Pass 1: one workgroup per radial
+// layer s (cells i = s + ng, s < n + ng, i.e. the interior and the outer ghosts) sums
+// rho dV and dV over the interior (j, k) in a fixed order.  Pass 2 (one thread): the
+// enclosed mass at each cell centre by the reference's sequential scan, then g = -G M / r^2.
+template <int ND, int GEOM, class S>
+__global__ __launch_bounds__(256) void k_shell_sums(GridDev G, const S* __restrict__ rho, double* __restrict__ shells, const DevFlag* run)
+{
+    __shared__ double lds[4];
+    if (run && !run->go()) return;
+    int const s = blockIdx.x;
+    int const i = s + G.ngh[0];
+    int const ny = G.n[1], nz = G.n[2];
+    int64_t const cnt = (int64_t)ny * nz;
+    double sm = 0, sv = 0;
+    for (int64_t t = threadIdx.x; t < cnt; t += 256) {
+        int const j = G.ngh[1] + (int)(t % ny);
+        int const k = G.ngh[2] + (int)(t / ny);
+        double const dv = cell_volume<ND, GEOM>(G, i, j, k);
+        sm += (double)rho[flat(G, i, j, k)] * dv;
+        sv += dv;
+    }
+    sm = block_reduce<kSum>(sm, lds, threadIdx.x, 256);
+    sv = block_reduce<kSum>(sv, lds, threadIdx.x, 256);
+    if (threadIdx.x == 0) {
+        shells[2 * s] = sm;
+        shells[2 * s + 1] = sv;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_mass_table(GridDev G, double central_mass, const double* __restrict__ shells, double* __restrict__ gtab,
+                                                   const DevFlag* run)
+{
+    if (threadIdx.x != 0) return;
+    if (run && !run->go()) return;
+    int const ng0 = G.ngh[0], n0 = G.n[0];
+    double const* x0 = G.x[0];
+    double const* xc = G.xc[0];
+    double const vol_s3 = 4 * kPi / 3;
+    double partial = 0;
+    for (int i = 0; i < G.nwg[0]; ++i) {
+        double M;
+        if (i < ng0) {
+            M = central_mass;
+        } else {
+            int const off = i - ng0;
+            double const x3 = x0[i] * x0[i] * x0[i];
+            double const rm = shells[2 * off] / shells[2 * off + 1];
+            if (i == ng0) {
+                partial += central_mass + (vol_s3 * ((xc[i] * xc[i] * xc[i]) - x3) * rm);
+            } else {
+                double const rmm = shells[2 * (off - 1)] / shells[2 * (off - 1) + 1];
+                partial += (vol_s3 * (x3 - (xc[i - 1] * xc[i - 1] * xc[i - 1])) * rmm) + (vol_s3 * ((xc[i] * xc[i] * xc[i]) - x3) * rm);
+            }
+            M = partial;
+        }
+        gtab[i] = -kGrav * M / (xc[i] * xc[i]);
+    }
+    (void)n0;
+}
+
+template <class S>
+void launch_internal_mass_gravity(int nd, int geom, const GridDev& G, double central_mass, const S* rho, double* shells, double* gtab,
+                                  const DevFlag* run, hipStream_t s)
+{
+    unsigned const nsh = (unsigned)(G.n[0] + G.ngh[0]);
+    HCLPP_GEOM_SWITCH(geom, HCLPP_ND_SWITCH(nd, hipLaunchKernelGGL((k_shell_sums<ND, GEOM, S>), dim3(nsh), dim3(256), 0, s, G, rho, shells, run)));
+    hipLaunchKernelGGL(k_mass_table, dim3(1), dim3(64), 0, s, G, central_mass, shells, gtab, run);
+}
+template void launch_internal_mass_gravity<double>(int, int, const GridDev&, double, const double*, double*, double*, const DevFlag*, hipStream_t);
+template void launch_internal_mass_gravity<float>(int, int, const GridDev&, double, const float*, double*, double*, const DevFlag*, hipStream_t);
+
 }  // namespace hclpp

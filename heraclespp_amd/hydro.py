@@ -203,6 +203,25 @@ class PointMassGravity:
         return self.g[i] if d == 0 else 0.0
 
 
+class InternalMassGravity:
+    """gravity/gravity.hpp:75-103, gravity.cpp:59-128: g(i) = -G M(<xc(i)) / xc(i)^2 along dim 0, with
+    M the central mass plus the mass of the shell-averaged density inside xc(i).  The table lives on
+    the device and is rebuilt from rho every step (main.cpp:309, :528); `refresh` does it on demand."""
+
+    def __init__(self, central_mass: float, grid: Grid):
+        self.central_mass = float(central_mass)
+        self.grid = grid
+
+    def refresh(self, ctx: "Context", rho):
+        """make_internal_mass_gravity(central_mass, grid, rho): rebuild ctx's table from rho (ghost-filled)."""
+        check(lib().hclpp_internal_mass_gravity(ctx.handle, ctx._chk(rho, (), "rho")))
+
+    def table(self, ctx: "Context") -> np.ndarray:
+        out = np.zeros(self.grid.Nx_local_wg[0])
+        check(lib().hclpp_gravity_table(ctx.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+
 def make_uniform_gravity(gx0: float, gx1: float, gx2: float) -> UniformGravity:
     return UniformGravity((gx0, gx1, gx2))
 
@@ -211,11 +230,17 @@ def make_point_mass_gravity(central_mass: float, grid: Grid) -> PointMassGravity
     return PointMassGravity(central_mass, grid)
 
 
+def make_internal_mass_gravity(central_mass: float, grid: Grid) -> InternalMassGravity:
+    return InternalMassGravity(central_mass, grid)
+
+
 def _gravity_key(g):
     if g is None:
         return ("Uniform", 0.0, 0.0, 0.0)
     if isinstance(g, PointMassGravity):
         return ("Point_mass", g.central_mass)
+    if isinstance(g, InternalMassGravity):
+        return ("Internal_mass", g.central_mass)
     return ("Uniform",) + tuple(g.g)
 
 
@@ -251,6 +276,9 @@ class Context:
         cfg.riemann = _lib.RIEMANN[riemann]
         if isinstance(gravity, PointMassGravity):
             cfg.gravity = _lib.GRAVITY["Point_mass"]
+            cfg.central_mass = gravity.central_mass
+        elif isinstance(gravity, InternalMassGravity):
+            cfg.gravity = _lib.GRAVITY["Internal_mass"]
             cfg.central_mass = gravity.central_mass
         else:
             cfg.gravity = _lib.GRAVITY["Uniform"]
@@ -498,3 +526,16 @@ class DistributedBoundaryCondition:
         nd = self.grid.ndim
         check(lib().hclpp_fill_ghosts(c.handle, c._chk(rho, (), "rho"), c._chk(rhou, (nd,), "rhou"),
                                       c._chk(E, (), "E"), c._fx(fx, (nfx,), "fx")))
+
+
+# ------------------------------------------------------------ pressure fix
+def pressure_fix(range_: Range, eos: PerfectGas, grid: Grid, dt: float, eps: float, rho, rhou, E, rho_rec, rhou_rec, E_rec,
+                 rho_new, rhou_new, E_new, gravity=None, bc=None):
+    """hydro/pressure_fix.hpp:19-172: E_new is updated in place (range no_ghosts in main.cpp:459-475)."""
+    c = grid.context(eos, gravity, bc)
+    nd = grid.ndim
+    check(lib().hclpp_pressure_fix(c.handle, ctypes.byref(range_._c()), float(dt), float(eps), c._chk(rho, (), "rho"),
+                                   c._chk(rhou, (nd,), "rhou"), c._chk(E, (), "E"), c._chk(rho_rec, (nd, 2), "rho_rec"),
+                                   c._chk(rhou_rec, (nd, nd, 2), "rhou_rec"), c._chk(E_rec, (nd, 2), "E_rec"),
+                                   c._chk(rho_new, (), "rho_new"), c._chk(rhou_new, (nd,), "rhou_new"),
+                                   c._chk(E_new, (), "E_new")))

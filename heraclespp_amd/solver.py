@@ -24,7 +24,7 @@ import torch
 
 from . import _lib
 from ._lib import check, lib
-from .hydro import (Context, Grid, PerfectGas, PointMassGravity, Range, UniformGravity)
+from .hydro import (Context, Grid, InternalMassGravity, PerfectGas, PointMassGravity, Range, UniformGravity)
 
 
 def _dev(a, shape, device="cuda"):
@@ -39,17 +39,21 @@ def _dev(a, shape, device="cuda"):
 class HydroSolver:
     def __init__(self, grid: Grid, eos: PerfectGas, *, reconstruction: str = "VanLeer", riemann_solver: str = "HLL",
                  gravity=None, bc=None, bc_order=None, cfl: float = 0.4, nfx: int = 0, fused: bool = True,
-                 precision: str = "fp64"):
+                 precision: str = "fp64", pressure_fix: str = "Off", eps_pf: float = 1e-6):
         if precision not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32' (FP32 storage with FP64 pressure recovery)")
-        if precision == "fp32" and (not fused or nfx):
-            raise ValueError("FP32 storage is a fused-step mode without passive scalars")
+        if pressure_fix not in ("On", "Off"):
+            raise ValueError("pressure_fix must be 'On' or 'Off'")
+        self.pressure_fix = pressure_fix == "On"  # main.cpp:459 (operator path: it needs the face states)
+        self.eps_pf = float(eps_pf)
+        if precision == "fp32" and (not fused or nfx or self.pressure_fix):
+            raise ValueError("FP32 storage is a fused-step mode without passive scalars or pressure fix")
         self.grid = grid
         self.eos = eos
         self.gravity = gravity
         self.cfl = float(cfl)
         self.nfx = nfx
-        self.fused = fused and nfx == 0
+        self.fused = fused and nfx == 0 and not self.pressure_fix
         self.precision = precision
         self.ctx = Context(grid, eos, gravity, bc, bc_order, nfx, reconstruction, riemann_solver, precision)
         nd = grid.ndim
@@ -114,6 +118,7 @@ class HydroSolver:
                                             c._chk(self.P), c._chk(m, (nd,)), c._chk(E)))
         check(lib().hclpp_fill_ghosts(c.handle, c._chk(r), c._chk(m, (nd,)), c._chk(E),
                                       c._fx(self.fx, (self.nfx,), "fx")))
+        self._refresh_gravity()  # main.cpp:309
         self.update_primitives()
         self.t = float(t0)
         self.iter = 0
@@ -217,6 +222,11 @@ class HydroSolver:
         check(lib().hclpp_clock_read(self.ctx.handle, ctypes.byref(out)))
         return out
 
+    def _refresh_gravity(self):
+        """make_gravity(param, grid, rho) of main.cpp:309/:528: only internal-mass gravity depends on rho."""
+        if isinstance(self.gravity, InternalMassGravity):
+            self.gravity.refresh(self.ctx, self.rho)
+
     def _step_operators(self, t_end, max_iter) -> bool:
         g = self.grid
         c = self.ctx
@@ -251,12 +261,18 @@ class HydroSolver:
                               c._chk(self.rhou_rec, (nd, nd, 2)), c._chk(self.E_rec, (nd, 2)),
                               c._fx(self.fx_rec, (nfx, nd, 2), "fx_rec"), c._chk(nxt[0]), c._chk(nxt[1], (nd,)),
                               c._chk(nxt[2]), c._fx(self.fx_new, (nfx,), "fx_new")))
+        if self.pressure_fix:  # :459-475
+            check(L.hclpp_pressure_fix(h, ctypes.byref(self._r(0)), dt, self.eps_pf, c._chk(self.rho),
+                                       c._chk(self.rhou, (nd,)), c._chk(self.E), c._chk(self.rho_rec, (nd, 2)),
+                                       c._chk(self.rhou_rec, (nd, nd, 2)), c._chk(self.E_rec, (nd, 2)), c._chk(nxt[0]),
+                                       c._chk(nxt[1], (nd,)), c._chk(nxt[2])))
         check(L.hclpp_fill_ghosts(h, c._chk(nxt[0]), c._chk(nxt[1], (nd,)), c._chk(nxt[2]),
                                   c._fx(self.fx_new, (nfx,), "fx_new")))
         self._cur = 1 - self._cur
         if nfx:
             self.fx, self.fx_new = self.fx_new, self.fx
         self.update_primitives()
+        self._refresh_gravity()  # :528
         self.t += dt
         self.iter += 1
         return exit_

@@ -24,6 +24,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "hclpp_hip.h"
@@ -178,6 +179,32 @@ public:
 class Grid;
 inline PointMassGravity make_point_mass_gravity(double central_mass, Grid const&) { return PointMassGravity(central_mass); }
 
+// gravity/gravity.hpp:75-103, gravity.cpp:59-128: radial gravity of the central mass plus the
+// shell-averaged density inside each cell centre.  The table is built on the device from `rho`
+// (ghost-filled), on the first operator call that uses this object on a given context; as in
+// main.cpp:309/:528 a new object is made from the current density after every step.
+class InternalMassGravity
+{
+public:
+    double central_mass;
+    double const* rho;
+    long long generation;
+    InternalMassGravity(double m, double const* r) : central_mass(m), rho(r), generation(next_generation()) {}
+
+private:
+    static long long next_generation()
+    {
+        static long long g = 0;
+        return ++g;
+    }
+};
+
+template <class View>
+InternalMassGravity make_internal_mass_gravity(double central_mass, Grid const&, View const& rho)
+{
+    return InternalMassGravity(central_mass, rho.data());
+}
+
 // ------------------------------------------------------------ mesh/grid.hpp:23-75 (one rank)
 class Grid
 {
@@ -261,8 +288,12 @@ public:
         return o;
     }
 
+    // generation of the internal-mass gravity table last built on a context
+    long long& gravity_generation(hclpp_ctx* c) const { return (*m_gen)[c]; }
+
 private:
     std::shared_ptr<std::map<CtxKey, std::shared_ptr<hclpp_ctx>>> m_ctx = std::make_shared<std::map<CtxKey, std::shared_ptr<hclpp_ctx>>>();
+    std::shared_ptr<std::map<hclpp_ctx*, long long>> m_gen = std::make_shared<std::map<hclpp_ctx*, long long>>();
 };
 
 namespace detail {
@@ -271,11 +302,23 @@ inline std::tuple<int, std::array<double, 3>, double> gravity_params(PointMassGr
 {
     return {HCLPP_GRAV_POINT_MASS, {0.0, 0.0, 0.0}, g.central_mass};
 }
+inline std::tuple<int, std::array<double, 3>, double> gravity_params(InternalMassGravity const& g)
+{
+    return {HCLPP_GRAV_INTERNAL_MASS, {0.0, 0.0, 0.0}, g.central_mass};
+}
 template <class Gravity>
 hclpp_ctx* ctx_for(Grid const& grid, PerfectGas const& eos, Gravity const& gravity, int nfx)
 {
     auto const [kind, g, m] = gravity_params(gravity);
-    return grid.context(eos, kind, g, m, {0, 0, 0, 0, 0, 0}, grid.default_order(), nfx);
+    hclpp_ctx* c = grid.context(eos, kind, g, m, {0, 0, 0, 0, 0, 0}, grid.default_order(), nfx);
+    if constexpr (std::is_same_v<Gravity, InternalMassGravity>) {
+        long long& gen = grid.gravity_generation(c);
+        if (gen != gravity.generation) {
+            check(hclpp_internal_mass_gravity(c, gravity.rho));
+            gen = gravity.generation;
+        }
+    }
+    return c;
 }
 inline hclpp_ctx* ctx_for(Grid const& grid, PerfectGas const& eos, int nfx = 0)
 {
@@ -458,6 +501,16 @@ inline double time_step(Range const& range, PerfectGas const& eos, Grid const& g
     double dt = 0;
     check(hclpp_time_step(detail::ctx_for(grid, eos), &r, rho.data(), u.data(), P.data(), &dt));
     return dt;
+}
+
+// hydro/pressure_fix.hpp:19-172 (E_new updated in place)
+inline void pressure_fix(Range const& range, PerfectGas const& eos, Grid const& grid, double dt, double eps, KV_double_3d const& rho,
+                         KV_double_4d const& rhou, KV_double_3d const& E, KV_double_5d const& rho_rec, KV_double_6d const& rhou_rec,
+                         KV_double_5d const& E_rec, KV_double_3d const& rho_new, KV_double_4d const& rhou_new, KV_double_3d const& E_new)
+{
+    hclpp_range const r = range.cells();
+    check(hclpp_pressure_fix(detail::ctx_for(grid, eos), &r, dt, eps, rho.data(), rhou.data(), E.data(), rho_rec.data(), rhou_rec.data(),
+                             E_rec.data(), rho_new.data(), rhou_new.data(), E_new.data()));
 }
 
 // analysis/integration.cpp:22-37

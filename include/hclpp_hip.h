@@ -39,7 +39,7 @@ extern "C" {
 enum { HCLPP_GEOM_CARTESIAN = 0, HCLPP_GEOM_SPHERICAL = 1 };
 enum { HCLPP_LIM_CONSTANT = 0, HCLPP_LIM_VANLEER = 1, HCLPP_LIM_MINMOD = 2, HCLPP_LIM_VANALBADA = 3 };
 enum { HCLPP_RS_HLL = 0, HCLPP_RS_HLLC = 1, HCLPP_RS_LOWMACH = 2 };
-enum { HCLPP_GRAV_UNIFORM = 0, HCLPP_GRAV_POINT_MASS = 1 };
+enum { HCLPP_GRAV_UNIFORM = 0, HCLPP_GRAV_POINT_MASS = 1, HCLPP_GRAV_INTERNAL_MASS = 2 };
 enum { HCLPP_BC_PERIODIC = 0, HCLPP_BC_NULLGRADIENT = 1, HCLPP_BC_REFLEXIVE = 2 };
 enum { HCLPP_FP64 = 0, HCLPP_FP32_STORAGE = 1 };
 
@@ -48,7 +48,7 @@ typedef struct hclpp_ctx hclpp_ctx;
 /* Replaces the compile-time ndim/geom/EoS/gravity choices and the run-time
  * Grid, PerfectGas, gravity and boundary objects built in main.cpp:140-303
  * (Grid: mesh/grid.cpp:48-76; PerfectGas(gamma, mmw): perfect_gas.cpp:13-22;
- * make_uniform_gravity / make_point_mass_gravity: gravity/gravity.cpp:29-55;
+ * make_uniform_gravity / make_point_mass_gravity / make_internal_mass_gravity: gravity/gravity.cpp:29-128;
  * bc choices and priority: main.cpp:175-186, boundary_distribute.cpp:33-71). */
 typedef struct {
     int ndim;          /* 1, 2 or 3 */
@@ -64,7 +64,7 @@ typedef struct {
     double gamma;      /* adiabatic index (> 1) */
     double mmw;        /* mean molecular weight (> 0) */
     double g[3];       /* uniform gravity */
-    double central_mass; /* point-mass gravity */
+    double central_mass; /* point-mass and internal-mass gravity */
     const double* x[3];  /* HOST node coordinates, nwg[d] + 1 entries (ghosts included) */
     int precision;     /* HCLPP_FP64 or HCLPP_FP32_STORAGE (fused step only) */
 } hclpp_config;
@@ -122,6 +122,18 @@ int hclpp_min_internal_energy(hclpp_ctx* ctx, const hclpp_range* range, const do
                               double* out);
 /* integrate: analysis/integration.cpp:22-37 (sum of var * dV).  Synchronises. */
 int hclpp_integrate(hclpp_ctx* ctx, const hclpp_range* range, const double* var, double* out);
+/* pressure_fix: hydro/pressure_fix.hpp:19-172 (main.cpp:459-475, range no_ghosts); rec arrays are the
+ * MUSCL-Hancock face states of the step, E_new is updated in place. */
+int hclpp_pressure_fix(hclpp_ctx* ctx, const hclpp_range* range, double dt, double eps, const double* rho, const double* rhou,
+                       const double* E, const double* rho_rec, const double* rhou_rec, const double* E_rec, const double* rho_new,
+                       const double* rhou_new, double* E_new);
+/* make_internal_mass_gravity: gravity/gravity.cpp:59-128 (one rank along dim 0).  Rebuilds the
+ * context's radial gravity table from rho (ghost-filled), as main.cpp:309 and :528 do after
+ * every step; the fused step does it by itself.  Needs gravity = HCLPP_GRAV_INTERNAL_MASS. */
+int hclpp_internal_mass_gravity(hclpp_ctx* ctx, const double* rho);
+/* Copy the context's radial gravity table g(x0 centre) (nwg[0] doubles, point-mass or internal-mass
+ * gravity) to host memory.  Synchronises. */
+int hclpp_gravity_table(hclpp_ctx* ctx, double* host_out);
 /* DistributedBoundaryCondition::operator() on one rank: boundary_distribute.hpp:42-76,
  * boundary_distribute.cpp:75-120, boundary.cpp:33-108 (conservative arrays). */
 int hclpp_fill_ghosts(hclpp_ctx* ctx, double* rho, double* rhou, double* E, double* fx);

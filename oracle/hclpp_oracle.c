@@ -867,6 +867,131 @@ void or_boundaries(const or_ctx* c, double* rho, double* rhou, double* E, double
     }
 }
 
+/* hydro/pressure_fix.hpp:19-172, over `r` (no_ghosts in main.cpp:459-475) */
+void or_pressure_fix(const or_ctx* c, const int* r, double dt, double eps, const double* rho, const double* rhou,
+                     const double* E, const double* rho_rec, const double* rhou_rec, const double* E_rec,
+                     const double* rho_new, const double* rhou_new, double* E_new)
+{
+    int const nd = c->nd;
+    int64_t const pl = c->plane;
+    double const eps2 = eps / 10;
+    FOR_RANGE(r)
+    {
+        int64_t const id = IDX(c, i, j, k);
+        cons_t cons, cons_new;
+        cons.rho = rho[id];
+        cons_new.rho = rho_new[id];
+        for (int d = 0; d < 3; ++d) {
+            cons.m[d] = d < nd ? rhou[id + pl * d] : 0;
+            cons_new.m[d] = d < nd ? rhou_new[id + pl * d] : 0;
+        }
+        cons.E = E[id];
+        cons_new.E = E_new[id];
+        prim_t const prim = to_prim(nd, &c->eos, &cons);
+        double const ekin = ek_cons(nd, &cons);
+        double const evol = E[id] - ekin;
+        double const evol_new = E_new[id] - ek_cons(nd, &cons_new);
+        double const de_god = evol_new - evol;
+        if (evol < eps * ekin) {
+            E_new[id] = E[id];
+            double divU = 0, divUE = 0, einL = 0, einR = 0;
+            double alpha = (evol * evol) / (ekin * ekin * eps * eps);
+            if (evol < eps2 * ekin) alpha = 0;
+            for (int idim = 0; idim < nd; ++idim) {
+                int const im = i - (idim == 0), jm = j - (idim == 1), km = k - (idim == 2);
+                int const ip = i + (idim == 0), jp = j + (idim == 1), kp = k + (idim == 2);
+                int64_t const idm = IDX(c, im, jm, km), idp = IDX(c, ip, jp, kp);
+                for (int ipos = 0; ipos < nd; ++ipos) {
+                    cons_t vL, vR, mR, pL;
+#define LOADF(q, cell, s)                                                                                \
+    do {                                                                                                 \
+        (q).rho = rho_rec[(cell) + pl * ((s) + 2 * idim)];                                               \
+        for (int v = 0; v < 3; ++v) (q).m[v] = v < nd ? rhou_rec[(cell) + pl * ((s) + 2 * (idim + nd * v))] : 0; \
+        (q).E = E_rec[(cell) + pl * ((s) + 2 * idim)];                                                   \
+    } while (0)
+                    LOADF(vL, id, 0);
+                    LOADF(vR, id, 1);
+                    LOADF(mR, idm, 1);
+                    LOADF(pL, idp, 0);
+#undef LOADF
+                    prim_t const primL = to_prim(nd, &c->eos, &vL);
+                    prim_t const primR = to_prim(nd, &c->eos, &vR);
+                    prim_t const prim_mR = to_prim(nd, &c->eos, &mR);
+                    prim_t const prim_pL = to_prim(nd, &c->eos, &pL);
+                    cons_t cons_p, cons_m;
+                    cons_p.rho = rho[idp];
+                    cons_m.rho = rho[idm];
+                    for (int d = 0; d < 3; ++d) {
+                        cons_p.m[d] = d < nd ? rhou[idp + pl * d] : 0;
+                        cons_m.m[d] = d < nd ? rhou[idm + pl * d] : 0;
+                    }
+                    cons_p.E = E[idp];
+                    cons_m.E = E[idm];
+                    double const us = (prim_mR.u[ipos] + primL.u[ipos]) / 2;   /* :129 */
+                    double const us_p = (primR.u[ipos] + prim_pL.u[ipos]) / 2; /* :130 */
+                    double const dsp = c->ds[idp + pl * ipos], ds0 = c->ds[id + pl * ipos], dv = c->dv[id];
+                    divU += ((us_p * dsp) - (us * ds0)) / dv;                  /* :132 */
+                    if (us_p > 0) einR = E[id] - ek_cons(nd, &cons);
+                    else einR = E[idp] - ek_cons(nd, &cons_p);
+                    if (us < 0) einL = E[id] - ek_cons(nd, &cons);
+                    else einL = E[idm] - ek_cons(nd, &cons_m);
+                    divUE += ((us_p * einR * dsp) - (us * einL * ds0)) / dv;   /* :146 */
+                    double const de_pf = dt * ((-prim.P * divU) - divUE);
+                    double sum = 0;
+                    for (int idr = 0; idr < nd; ++idr) sum += cons.m[idr] * cons.m[idr] / cons.rho;
+                    double const de_tot = (alpha * de_god) + ((1 - alpha) * de_pf);
+                    E_new[id] = evol + de_tot + (1. / 2 * sum);              /* :160 */
+                }
+            }
+        }
+    }
+}
+
+/* gravity/gravity.cpp:59-128 (one rank along dim 0): shell-averaged density of each radial
+ * layer (interior j, k), enclosed mass at each cell centre, g = -G M(r) / r^2 */
+void or_internal_mass_gravity(or_ctx* c, const double* rho)
+{
+    int const ng0 = c->ngh[0], n0 = c->n[0];
+    int const nsh = n0 + ng0;
+    double* mass = (double*)xcalloc(nsh, sizeof(double));
+    double* vol = (double*)xcalloc(nsh, sizeof(double));
+    double* Mr = (double*)xcalloc(c->nwg[0], sizeof(double));
+    for (int s = 0; s < nsh; ++s) { /* :75-87, sums in k-major, j-minor order */
+        int const i = s + ng0;
+        double sm = 0, sv = 0;
+        for (int k = c->ngh[2]; k < c->ngh[2] + c->n[2]; ++k)
+            for (int j = c->ngh[1]; j < c->ngh[1] + c->n[1]; ++j) {
+                int64_t const id = IDX(c, i, j, k);
+                sm += rho[id] * c->dv[id];
+                sv += c->dv[id];
+            }
+        mass[s] = sm;
+        vol[s] = sv;
+    }
+    for (int s = 0; s < nsh; ++s) mass[s] /= vol[s];                       /* :96-99 rho_mean */
+    for (int i = 0; i < ng0; ++i) Mr[i] = c->cfg.central_mass;             /* :101 */
+    double partial = 0;                                                     /* :103-118 */
+    double const vol_s3 = 4 * OR_PI / 3;
+    for (int i = ng0; i < n0 + 2 * ng0; ++i) {
+        int const off = i - ng0;
+        double const *x0 = c->x[0], *xc = c->xc[0];
+        double const x3 = x0[i] * x0[i] * x0[i];
+        if (i == ng0) {
+            partial += c->cfg.central_mass + (vol_s3 * ((xc[i] * xc[i] * xc[i]) - x3) * mass[off]);
+        } else {
+            partial += (vol_s3 * (x3 - (xc[i - 1] * xc[i - 1] * xc[i - 1])) * mass[off - 1])
+                       + (vol_s3 * ((xc[i] * xc[i] * xc[i]) - x3) * mass[off]);
+        }
+        Mr[i] = partial;
+    }
+    for (int i = 0; i < c->nwg[0]; ++i) c->gpm[i] = -OR_G * Mr[i] / (c->xc[0][i] * c->xc[0][i]); /* :120-123 */
+    free(mass);
+    free(vol);
+    free(Mr);
+}
+
+const double* or_gravity_table(const or_ctx* c) { return c->gpm; }
+
 /* ------------------------------------------------------------------ */
 /* driver: main.cpp:305-311 (init) and :354-531 (loop body) */
 int or_init_from_prim(or_ctx* c, const double* rho, const double* u, const double* P, const double* fx)
@@ -882,6 +1007,7 @@ int or_init_from_prim(or_ctx* c, const double* rho, const double* u, const doubl
     or_range(c, c->ng, rall);
     or_conv_prim_to_cons(c, r0, c->rho, c->u, c->P, c->rhou, c->E);
     or_boundaries(c, c->rho, c->rhou, c->E, c->fx);
+    if (c->cfg.gravity == OR_GRAV_INTERNAL_MASS) or_internal_mass_gravity(c, c->rho); /* :309 */
     or_conv_cons_to_prim(c, rall, c->rho, c->rhou, c->E, c->u, c->P);
     c->t = 0;
     c->iter = 0;
@@ -906,12 +1032,16 @@ int or_step(or_ctx* c, double cfl, double t_end, int max_iter, double* dt_out, i
                             c->P_rec, c->u_rec);                   /* :427-439 */
     or_godunov(c, r0, dt, c->rho, c->rhou, c->E, c->fx, c->rho_rec, c->rhou_rec, c->E_rec, c->fx_rec, c->rho_new,
                c->rhou_new, c->E_new, c->fx_new);                   /* :441-457 */
+    if (c->cfg.pressure_fix)                                            /* :459-475 */
+        or_pressure_fix(c, r0, dt, c->cfg.eps_pf, c->rho, c->rhou, c->E, c->rho_rec, c->rhou_rec, c->E_rec, c->rho_new,
+                        c->rhou_new, c->E_new);
     or_boundaries(c, c->rho_new, c->rhou_new, c->E_new, c->fx_new); /* :479 */
     memcpy(c->rho, c->rho_new, sizeof(double) * pl);                /* :481-484 */
     memcpy(c->rhou, c->rhou_new, sizeof(double) * pl * nd);
     memcpy(c->E, c->E_new, sizeof(double) * pl);
     if (c->nfx > 0) memcpy(c->fx, c->fx_new, sizeof(double) * pl * c->nfx);
     or_conv_cons_to_prim(c, rall, c->rho, c->rhou, c->E, c->u, c->P); /* :526 */
+    if (c->cfg.gravity == OR_GRAV_INTERNAL_MASS) or_internal_mass_gravity(c, c->rho); /* :528 */
     c->t += dt;                                                        /* :530-531 */
     c->iter += 1;
     if (dt_out) *dt_out = dt;

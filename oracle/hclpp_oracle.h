@@ -24,7 +24,7 @@ extern "C" {
 enum { OR_GEOM_CARTESIAN = 0, OR_GEOM_SPHERICAL = 1 };
 enum { OR_LIM_CONSTANT = 0, OR_LIM_VANLEER = 1, OR_LIM_MINMOD = 2, OR_LIM_VANALBADA = 3 };
 enum { OR_RS_HLL = 0, OR_RS_HLLC = 1, OR_RS_LOWMACH = 2 };
-enum { OR_GRAV_UNIFORM = 0, OR_GRAV_POINT_MASS = 1 };
+enum { OR_GRAV_UNIFORM = 0, OR_GRAV_POINT_MASS = 1, OR_GRAV_INTERNAL_MASS = 2 };
 enum { OR_BC_PERIODIC = 0, OR_BC_NULLGRADIENT = 1, OR_BC_REFLEXIVE = 2 };
 
 typedef struct {
@@ -42,6 +42,8 @@ typedef struct {
     double g[3];       /* uniform gravity */
     double central_mass;
     const double* x[3]; /* node coordinates, nwg[d]+1 entries each (host) */
+    int pressure_fix;   /* main.cpp:459: "Pressure fix" = On */
+    double eps_pf;      /* param.cpp:71 (default 1e-6) */
 } or_config;
 
 typedef struct or_ctx or_ctx;
@@ -84,6 +86,13 @@ double or_time_step(const or_ctx* c, const int* range, const double* rho, const 
 double or_min_internal_energy(const or_ctx* c, const int* range, const double* rho, const double* rhou, const double* E);
 double or_integrate(const or_ctx* c, const int* range, const double* var);
 void or_boundaries(const or_ctx* c, double* rho, double* rhou, double* E, double* fx);
+/* hydro/pressure_fix.hpp:19-172 (updates E_new in place) */
+void or_pressure_fix(const or_ctx* c, const int* range, double dt, double eps, const double* rho, const double* rhou,
+                     const double* E, const double* rho_rec, const double* rhou_rec, const double* E_rec,
+                     const double* rho_new, const double* rhou_new, double* E_new);
+/* gravity/gravity.cpp:59-128: rebuild the radial gravity table from rho (ghost-filled); g table out (nwg[0]) */
+void or_internal_mass_gravity(or_ctx* c, const double* rho);
+const double* or_gravity_table(const or_ctx* c);
 void or_range(const or_ctx* c, int nghost_eff, int* range6);
 
 /* driver (main.cpp:305-311, 354-531) on the context's own arrays */

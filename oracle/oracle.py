@@ -24,7 +24,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 LIMITERS = {"Constant": 0, "VanLeer": 1, "Minmod": 2, "VanAlbada": 3}
 SOLVERS = {"HLL": 0, "HLLC": 1, "Low Mach": 2}
 GEOMS = {"Cartesian": 0, "Spherical": 1}
-GRAVITIES = {"Uniform": 0, "Point_mass": 1}
+GRAVITIES = {"Uniform": 0, "Point_mass": 1, "Internal_mass": 2}
 BCS = {"Periodic": 0, "NullGradient": 1, "Reflexive": 2}
 
 
@@ -45,6 +45,8 @@ class _Cfg(ctypes.Structure):
         ("g", ctypes.c_double * 3),
         ("central_mass", ctypes.c_double),
         ("x", ctypes.POINTER(ctypes.c_double) * 3),
+        ("pressure_fix", ctypes.c_int),
+        ("eps_pf", ctypes.c_double),
     ]
 
 
@@ -106,6 +108,10 @@ def lib():
         L.or_min_internal_energy.restype = ctypes.c_double
         L.or_min_internal_energy.argtypes = [vp, ip, dp, dp, dp]
         L.or_boundaries.argtypes = [vp, dp, dp, dp, dp]
+        L.or_pressure_fix.argtypes = [vp, ip, ctypes.c_double, ctypes.c_double] + [dp] * 9
+        L.or_internal_mass_gravity.argtypes = [vp, dp]
+        L.or_gravity_table.restype = dp
+        L.or_gravity_table.argtypes = [vp]
         L.or_init_from_prim.restype = ctypes.c_int
         L.or_init_from_prim.argtypes = [vp, dp, dp, dp, dp]
         L.or_step.restype = ctypes.c_int
@@ -201,7 +207,8 @@ class Oracle:
 
     def __init__(self, ndim, n, *, geom="Cartesian", limiter="Minmod", riemann="HLL", gamma=1.4, mmw=1.0,
                  gravity="Uniform", g=(0.0, 0.0, 0.0), central_mass=1.0, nfx=0, ng=2,
-                 xmin=(0.0, 0.0, 0.0), xmax=(1.0, 1.0, 1.0), nodes=None, bc=None, bc_order=None):
+                 xmin=(0.0, 0.0, 0.0), xmax=(1.0, 1.0, 1.0), nodes=None, bc=None, bc_order=None,
+                 pressure_fix=False, eps_pf=1e-6):
         self.ndim = ndim
         self.n = [int(v) for v in (list(n) + [1, 1, 1])[:3]]
         for d in range(ndim, 3):
@@ -230,6 +237,8 @@ class Oracle:
         cfg.gamma = gamma
         cfg.mmw = mmw
         cfg.central_mass = central_mass
+        cfg.pressure_fix = 1 if pressure_fix else 0
+        cfg.eps_pf = eps_pf
         self._cfg = cfg
         h = lib().or_create(ctypes.byref(cfg))
         if not h:
@@ -299,6 +308,18 @@ class Oracle:
 
     def boundaries(self, rho, rhou, E, fx):
         lib().or_boundaries(self._h, _p(rho), _p(rhou), _p(E), _p(fx))
+
+    def pressure_fix(self, ngeff, dt, eps, rho, rhou, E, rho_rec, rhou_rec, E_rec, rho_new, rhou_new, E_new):
+        lib().or_pressure_fix(self._h, self.range(ngeff), dt, eps, _p(rho), _p(rhou), _p(E), _p(rho_rec), _p(rhou_rec),
+                              _p(E_rec), _p(rho_new), _p(rhou_new), _p(E_new))
+
+    def internal_mass_gravity(self, rho):
+        """Rebuild the radial gravity table from rho (ghost-filled); returns a copy of the table."""
+        lib().or_internal_mass_gravity(self._h, _p(rho))
+        return self.gravity_table()
+
+    def gravity_table(self):
+        return np.ctypeslib.as_array(lib().or_gravity_table(self._h), shape=(self.nwg[0],)).copy()
 
     # ---- driver
     def init_from_prim(self, rho, u, P, fx=None):

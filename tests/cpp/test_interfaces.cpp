@@ -196,12 +196,167 @@ static void test_operator_loop()
     or_destroy(o);
 }
 
+// main.cpp:305-531 with internal-mass gravity (rebuilt from rho after every step, :309, :528),
+// 1D spherical, against the oracle driver (one cell per shell: the shell sums are exact).
+static void test_internal_mass_loop()
+{
+    int const nd = 1, n = 64, nsteps = 20;
+    double const cfl = 0.4, gamma = 1.4, M = 2.0e10;
+    Grid const grid(nd, {n, 1, 1}, 2, Geometry::Geom_spherical, {0.5, 0, 0}, {1.5, 1, 1});
+    PerfectGas const eos(gamma, 1.);
+    auto const& w = grid.Nx_local_wg;
+    size_t const cells = static_cast<size_t>(w[0]);
+    std::vector<double> hrho(cells, 0.), hu(cells, 0.), hP(cells, 0.);
+    for (int i = 2; i < n + 2; ++i) {
+        double const r = 0.5 + (i - 1.5) / n;
+        hrho[i] = 1.0 + 0.5 * std::exp(-(r - 1.0) * (r - 1.0) / 0.01);
+        hu[i] = 0.1 * std::sin(9.0 * r);
+        hP[i] = 1.0 + 0.1 * std::cos(5.0 * r);
+    }
+    or_config oc {};
+    oc.ndim = nd;
+    oc.geom = OR_GEOM_SPHERICAL;
+    oc.limiter = OR_LIM_VANLEER;
+    oc.riemann = OR_RS_HLL;
+    oc.gravity = OR_GRAV_INTERNAL_MASS;
+    oc.central_mass = M;
+    oc.ng = 2;
+    oc.n[0] = n;
+    oc.n[1] = 1;
+    oc.n[2] = 1;
+    oc.bc[0] = OR_BC_REFLEXIVE;
+    oc.bc[1] = OR_BC_NULLGRADIENT;
+    oc.bc_order[0] = 0;
+    oc.bc_order[1] = 1;
+    oc.gamma = gamma;
+    oc.mmw = 1.;
+    for (int d = 0; d < 3; ++d) oc.x[d] = grid.x[d].data();
+    or_ctx* o = or_create(&oc);
+    or_init_from_prim(o, hrho.data(), hu.data(), hP.data(), nullptr);
+    for (int s = 0; s < nsteps; ++s) {
+        double dt;
+        int ex;
+        or_step(o, cfl, 1e30, nsteps, &dt, &ex);
+    }
+    KV_double_3d rho("rho", w[0], 1, 1), P("P", w[0], 1, 1), E("E", w[0], 1, 1);
+    KV_double_4d u("u", w[0], 1, 1, nd), rhou("rhou", w[0], 1, 1, nd), fx("fx", w[0], 1, 1, 0);
+    KV_double_3d rho_new("rho_new", w[0], 1, 1), E_new("E_new", w[0], 1, 1);
+    KV_double_4d rhou_new("rhou_new", w[0], 1, 1, nd), fx_new("fx_new", w[0], 1, 1, 0);
+    KV_double_5d rho_rec("rho_rec", w[0], 1, 1, 2, nd), E_rec("E_rec", w[0], 1, 1, 2, nd), P_rec("P_rec", w[0], 1, 1, 2, nd);
+    KV_double_6d rhou_rec("rhou_rec", w[0], 1, 1, 2, nd, nd), u_rec("u_rec", w[0], 1, 1, 2, nd, nd), fx_rec("fx_rec", w[0], 1, 1, 2, nd, 0);
+    rho.copy_from_host(hrho);
+    u.copy_from_host(hu);
+    P.copy_from_host(hP);
+    DistributedBoundaryCondition const bcs(grid, {"Reflexive", "NullGradient"});
+    auto reconstruction = std::make_unique<MUSCLHancockHydroReconstruction<InternalMassGravity>>(
+            factory_face_reconstruction("VanLeer"), std::make_unique<HancockExtrapolationReconstruction<InternalMassGravity>>(eos), eos, P_rec,
+            u_rec);
+    auto godunov = factory_godunov_scheme<InternalMassGravity>("HLL", eos);
+    conv_prim_to_cons(grid.range.no_ghosts(), eos, grid, rho, u, P, rhou, E);  // main.cpp:305
+    bcs(grid, rho, rhou, E, fx);                                                 // :307
+    auto g = make_internal_mass_gravity(M, grid, rho);                          // :309
+    conv_cons_to_prim(grid.range.all_ghosts(), eos, grid, rho, rhou, E, u, P);  // :311
+    for (int s = 0; s < nsteps; ++s) {
+        double const dt = cfl * time_step(grid.range.all_ghosts(), eos, grid, rho, u, P);
+        reconstruction->execute(grid.range.with_ghosts(1), grid, g, dt / 2, rho, u, P, fx, rho_rec, rhou_rec, E_rec, fx_rec);
+        godunov->execute(grid.range.no_ghosts(), grid, g, dt, rho, rhou, E, fx, rho_rec, rhou_rec, E_rec, fx_rec, rho_new, rhou_new, E_new, fx_new);
+        bcs(grid, rho_new, rhou_new, E_new, fx_new);
+        std::swap(rho, rho_new);
+        std::swap(rhou, rhou_new);
+        std::swap(E, E_new);
+        conv_cons_to_prim(grid.range.all_ghosts(), eos, grid, rho, rhou, E, u, P);
+        g = make_internal_mass_gravity(M, grid, rho);  // :528
+    }
+    auto const r = rho.to_host();
+    auto const m = rhou.to_host();
+    auto const e = E.to_host();
+    EXPECT(std::memcmp(r.data(), or_state(o, 0), r.size() * 8) == 0, "rho differs from the oracle (internal mass)");
+    EXPECT(std::memcmp(m.data(), or_state(o, 1), m.size() * 8) == 0, "rhou differs from the oracle (internal mass)");
+    EXPECT(std::memcmp(e.data(), or_state(o, 2), e.size() * 8) == 0, "E differs from the oracle (internal mass)");
+    or_destroy(o);
+}
+
+// pressure_fix after the Godunov update (main.cpp:459-475) on a cold uniform 2D stream, against the oracle.
+static void test_pressure_fix_loop()
+{
+    int const nd = 2, n = 32, nsteps = 10;
+    double const cfl = 0.4, gamma = 1.4, eps_pf = 1e-5;
+    Grid const grid(nd, {n, n, 1}, 2, Geometry::Geom_cartesian, {0, 0, 0}, {1, 1, 1});
+    PerfectGas const eos(gamma, 1.);
+    auto const& w = grid.Nx_local_wg;
+    size_t const cells = static_cast<size_t>(w[0]) * w[1];
+    std::vector<double> hrho(cells, 1.), hu(cells * nd, 3.), hP(cells, 0.);
+    for (size_t c = 0; c < cells; ++c) hP[c] = 1e-5 * (1.0 + 0.5 * std::sin(0.37 * c) * std::sin(0.11 * c));
+    or_config oc {};
+    oc.ndim = nd;
+    oc.geom = OR_GEOM_CARTESIAN;
+    oc.limiter = OR_LIM_VANLEER;
+    oc.riemann = OR_RS_HLL;
+    oc.gravity = OR_GRAV_UNIFORM;
+    oc.ng = 2;
+    oc.n[0] = n;
+    oc.n[1] = n;
+    oc.n[2] = 1;
+    for (int f = 0; f < 6; ++f) {
+        oc.bc[f] = OR_BC_PERIODIC;
+        oc.bc_order[f] = f;
+    }
+    oc.gamma = gamma;
+    oc.mmw = 1.;
+    oc.pressure_fix = 1;
+    oc.eps_pf = eps_pf;
+    for (int d = 0; d < 3; ++d) oc.x[d] = grid.x[d].data();
+    or_ctx* o = or_create(&oc);
+    or_init_from_prim(o, hrho.data(), hu.data(), hP.data(), nullptr);
+    for (int s = 0; s < nsteps; ++s) {
+        double dt;
+        int ex;
+        or_step(o, cfl, 1e30, nsteps, &dt, &ex);
+    }
+    UniformGravity const g({0.0, 0.0, 0.0});
+    KV_double_3d rho("rho", w[0], w[1], 1), P("P", w[0], w[1], 1), E("E", w[0], w[1], 1);
+    KV_double_4d u("u", w[0], w[1], 1, nd), rhou("rhou", w[0], w[1], 1, nd), fx("fx", w[0], w[1], 1, 0);
+    KV_double_3d rho_new("rho_new", w[0], w[1], 1), E_new("E_new", w[0], w[1], 1);
+    KV_double_4d rhou_new("rhou_new", w[0], w[1], 1, nd), fx_new("fx_new", w[0], w[1], 1, 0);
+    KV_double_5d rho_rec("rho_rec", w[0], w[1], 1, 2, nd), E_rec("E_rec", w[0], w[1], 1, 2, nd), P_rec("P_rec", w[0], w[1], 1, 2, nd);
+    KV_double_6d rhou_rec("rhou_rec", w[0], w[1], 1, 2, nd, nd), u_rec("u_rec", w[0], w[1], 1, 2, nd, nd),
+            fx_rec("fx_rec", w[0], w[1], 1, 2, nd, 0);
+    rho.copy_from_host(hrho);
+    u.copy_from_host(hu);
+    P.copy_from_host(hP);
+    DistributedBoundaryCondition const bcs(grid, {"Periodic", "Periodic", "Periodic", "Periodic"});
+    auto reconstruction = std::make_unique<MUSCLHancockHydroReconstruction<UniformGravity>>(
+            factory_face_reconstruction("VanLeer"), std::make_unique<HancockExtrapolationReconstruction<UniformGravity>>(eos), eos, P_rec, u_rec);
+    auto godunov = factory_godunov_scheme<UniformGravity>("HLL", eos);
+    conv_prim_to_cons(grid.range.no_ghosts(), eos, grid, rho, u, P, rhou, E);
+    bcs(grid, rho, rhou, E, fx);
+    conv_cons_to_prim(grid.range.all_ghosts(), eos, grid, rho, rhou, E, u, P);
+    for (int s = 0; s < nsteps; ++s) {
+        double const dt = cfl * time_step(grid.range.all_ghosts(), eos, grid, rho, u, P);
+        reconstruction->execute(grid.range.with_ghosts(1), grid, g, dt / 2, rho, u, P, fx, rho_rec, rhou_rec, E_rec, fx_rec);
+        godunov->execute(grid.range.no_ghosts(), grid, g, dt, rho, rhou, E, fx, rho_rec, rhou_rec, E_rec, fx_rec, rho_new, rhou_new, E_new, fx_new);
+        pressure_fix(grid.range.no_ghosts(), eos, grid, dt, eps_pf, rho, rhou, E, rho_rec, rhou_rec, E_rec, rho_new, rhou_new, E_new);
+        bcs(grid, rho_new, rhou_new, E_new, fx_new);
+        std::swap(rho, rho_new);
+        std::swap(rhou, rhou_new);
+        std::swap(E, E_new);
+        conv_cons_to_prim(grid.range.all_ghosts(), eos, grid, rho, rhou, E, u, P);
+    }
+    auto const e = E.to_host();
+    auto const r = rho.to_host();
+    EXPECT(std::memcmp(r.data(), or_state(o, 0), r.size() * 8) == 0, "rho differs from the oracle (pressure fix)");
+    EXPECT(std::memcmp(e.data(), or_state(o, 2), e.size() * 8) == 0, "E differs from the oracle (pressure fix)");
+    or_destroy(o);
+}
+
 int main()
 {
     try {
         test_errors();
         test_conversions();
         test_operator_loop();
+        test_internal_mass_loop();
+        test_pressure_fix_loop();
     } catch (std::exception const& e) {
         std::printf("FAIL: exception %s\n", e.what());
         return 2;
