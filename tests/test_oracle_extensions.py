@@ -214,3 +214,122 @@ def test_pressure_fix_uses_face_areas_of_the_shifted_cell():
     div = U * (A[1:] - A[:-1]) / V
     want = e0 - dt * (0.4 * e0 + e0) * div + ek[0, 0, 2:12]
     np.testing.assert_allclose(E_new[0, 0, 2:12], want, rtol=1e-13, atol=0)
+
+
+# ---------------------------------------------------------------- passive scalars
+# Closed forms for the scalar path of the predictor (extrapolation_reconstruction.hpp:87-108,
+# 315-337) and of the Godunov update (godunov_scheme.hpp:122, 263-301).  With a uniform density,
+# velocity U and pressure, the mass flux of every face is rho U, and the scalar obeys the
+# textbook schemes for linear advection with Courant number nu = U dt / dx (Toro, "Riemann
+# Solvers and Numerical Methods for Fluid Dynamics", ch. 13-14):
+#   Constant limiter: first-order upwind,  y_i' = y_i - nu (y_i - y_{i-1})            (U > 0)
+#   Minmod: MUSCL-Hancock, face y_R = y_i + (1 - nu) D_i / 2, D_i = minmod(y_{i+1} - y_i, y_i - y_{i-1}),
+#           y_i' = y_i - nu (y_R(i) - y_R(i-1))                                          (U > 0)
+# and the mirrored forms for U < 0.
+
+def _uniform_flow_1d(n, U, limiter, y):
+    o = orc.Oracle(1, [n], limiter=limiter, riemann="HLL", nfx=1, bc=["Periodic", "Periodic"])
+    rho = np.ones(o.shape3)
+    u = np.full((1,) + o.shape3, U)
+    P = np.ones(o.shape3)
+    fx = np.zeros((1,) + o.shape3)
+    fx[0, 0, 0, o.ng:o.ng + n] = y
+    fx[0, 0, 0, :o.ng] = y[-o.ng:]
+    fx[0, 0, 0, o.ng + n:] = y[:o.ng]
+    o.init_from_prim(rho, u, P, fx)
+    dt, _ = o.step(0.4, 1e30)
+    return o, dt
+
+
+def _minmod(a, b):
+    return np.where(a * b > 0, np.sign(a) * np.minimum(np.abs(a), np.abs(b)), 0.0)
+
+
+@pytest.mark.parametrize("U", [0.7, -0.7])
+def test_passive_scalar_first_order_upwind(U):
+    n = 32
+    y = np.where((np.arange(n) >= 8) & (np.arange(n) < 20), 0.7, 0.2)
+    o, dt = _uniform_flow_1d(n, U, "Constant", y)
+    nu = U * dt * n
+    if U > 0:
+        want = y - nu * (y - np.roll(y, 1))
+    else:
+        want = y - nu * (np.roll(y, -1) - y)
+    np.testing.assert_allclose(o.state("fx")[0, 0, 0, 2:2 + n], want, rtol=0, atol=2e-15)
+
+
+@pytest.mark.parametrize("U", [0.7, -0.7])
+def test_passive_scalar_muscl_hancock(U):
+    n = 40
+    x = (np.arange(n) + 0.5) / n
+    y = 0.5 + 0.3 * np.sin(2 * np.pi * x) + 0.1 * (x > 0.6)  # smooth part, a jump and extrema
+    o, dt = _uniform_flow_1d(n, U, "Minmod", y)
+    nu = abs(U) * dt * n
+    D = _minmod(np.roll(y, -1) - y, y - np.roll(y, 1))
+    if U > 0:
+        yr = y + (1 - nu) * D / 2  # upwind face value at i+1/2
+        want = y - nu * (yr - np.roll(yr, 1))
+    else:
+        yl = y - (1 - nu) * D / 2  # upwind face value at i-1/2
+        want = y - nu * (yl - np.roll(yl, -1))
+    np.testing.assert_allclose(o.state("fx")[0, 0, 0, 2:2 + n], want, rtol=0, atol=2e-15)
+
+
+def _random_flow_with_scalar(nd, n, geom, seed):
+    lo = SPH_LO if geom == "Spherical" else (0.0, 0.0, 0.0)
+    hi = SPH_HI if geom == "Spherical" else (1.0, 1.0, 1.0)
+    bc = (["Reflexive", "Reflexive", "Reflexive", "Reflexive", "Periodic", "Periodic"] if geom == "Spherical"
+          else ["Periodic"] * 6)[:2 * nd]
+    o = orc.Oracle(nd, n, geom=geom, xmin=lo, xmax=hi, limiter="VanLeer", riemann="HLLC", nfx=1, bc=bc)
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 0.3 * rng.random(o.shape3)
+    u = 0.3 * rng.standard_normal((nd,) + o.shape3)
+    P = 1.0 + 0.2 * rng.random(o.shape3)
+    return o, rho, u, P, rng
+
+
+SPH_LO = (0.3, 0.2, 0.0)
+SPH_HI = (1.3, np.pi - 0.2, 2 * np.pi)
+
+
+@pytest.mark.parametrize("nd,n,geom", [(1, [24], "Cartesian"), (2, [16, 12], "Cartesian"), (3, [10, 8, 6], "Cartesian"),
+                                       (2, [12, 8], "Spherical"), (3, [8, 6, 8], "Spherical")])
+def test_passive_scalar_of_one_follows_the_density(nd, n, geom):
+    """y = 1 everywhere: the partial density y rho takes exactly the density's updates (same
+    fluxes, same order), so y stays exactly 1 in every cell after several steps."""
+    o, rho, u, P, _ = _random_flow_with_scalar(nd, n, geom, seed=nd)
+    o.init_from_prim(rho, u, P, np.ones((1,) + o.shape3))
+    for _ in range(4):
+        o.step(0.4, 1e30)
+    np.testing.assert_array_equal(o.state("fx"), 1.0)
+
+
+@pytest.mark.parametrize("nd,n", [(1, [48]), (2, [20, 16]), (3, [10, 8, 6])])
+def test_passive_scalar_mass_is_conserved(nd, n):
+    """Periodic box, random flow and scalar in (0.2, 0.8) (no clamping): the scalar mass
+    sum(y rho dV) is conserved to rounding, which needs the same upwinded flux on both sides
+    of every face."""
+    o, rho, u, P, rng = _random_flow_with_scalar(nd, n, "Cartesian", seed=10 + nd)
+    y = 0.2 + 0.6 * rng.random((1,) + o.shape3)
+    o.init_from_prim(rho, u, P, y)
+    dv = o.geometry(7)
+    it = _interior(o)
+
+    def scalar_mass():
+        return float(np.sum((o.state("fx")[0] * o.state("rho") * dv)[it]))
+
+    m0 = scalar_mass()
+    for _ in range(5):
+        o.step(0.4, 1e30)
+        fx = o.state("fx")[0][it]
+        assert fx.min() > 0 and fx.max() < 1
+    assert abs(scalar_mass() - m0) <= 1e-14 * abs(m0)
+
+
+def test_passive_scalar_is_clamped():
+    """A scalar outside [0, 1] on input comes out clamped (godunov_scheme.hpp:294-301)."""
+    n = 16
+    y = np.linspace(-0.5, 1.5, n)
+    o, _ = _uniform_flow_1d(n, 0.3, "Constant", y)
+    fx = o.state("fx")[0, 0, 0, 2:2 + n]
+    assert fx.min() == 0.0 and fx.max() == 1.0
