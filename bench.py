@@ -16,9 +16,16 @@ calls.  Prints one JSON line:
              against the 8 TB/s HBM3E peak; `traffic` is the measured
              HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled
              by a copy-kernel calibration) read from profiles/, or null
+             roofline.valu: the FP64 kernel is bound by its VALU issue rate, not
+             by HBM; from the committed counter summary (SQ_ACTIVE_INST_VALU per
+             cell) and the live kernel time, the fraction of the 1024 SIMDs'
+             issue slots used and the VALU roof in Gcell/s
   cpu_baseline  the CPU oracle (a restatement of the reference's unfused
              kernels, OpenMP) on a bounded sample of the same workload, timed
              on this host's cores
+  adjoint    differentiable-Euler 1024^2 (VanAlbada + HLLC, 400 steps,
+             checkpoint every 20): forward, checkpointed backward and reverse-
+             kernel cost per step, and central-difference checks of the gradient
 """
 
 from __future__ import annotations
@@ -60,6 +67,16 @@ def _traffic(workload: str, precision: str):
     if not os.path.exists(path):
         return None
     m = re.search(r"^traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def _valu_units(workload: str, precision: str):
+    """VALU issue units per interior cell of the fused kernel (SQ_ACTIVE_INST_VALU x 64 / cells) from the
+    committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"k_step_march_{workload}_{precision}_valu.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"VALU issue units \(SQ_ACTIVE_INST_VALU\)\s+([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -129,9 +146,9 @@ def adjoint_block(n: int, steps: int, every: int, fd_dirs: int):
     dg = torch.zeros(1, dtype=torch.float64, device="cuda")
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     eng.vjp(U, lam, lam2, dg, prob.dts[0])
+    lams = [lam if k % 2 == 0 else lam2 for k in range(steps + 1)]
     e0.record()
-    for k in range(steps):
-        eng.vjp(U, lam, lam2, dg, prob.dts[k])
+    eng.vjps([U] * steps, lams, dg, prob.dts[:steps])  # one C call: no per-step host overhead
     e1.record()
     torch.cuda.synchronize()
     t_rev = e0.elapsed_time(e1) / 1e3
@@ -246,6 +263,14 @@ def main():
     bytes_alg = 2 * (s.ndim + 2) * word * cells
     achieved = bytes_alg / kernel_s / 1e9
     traffic = _traffic(a.workload, a.precision)
+    units = _valu_units(a.workload, a.precision)
+    # FP64 VALU issue capacity: 1024 SIMDs, one wave64 instruction per 4 cycles each, at 2.4 GHz
+    valu = None
+    if units is not None:
+        issued = units * cells / 64
+        capacity = 1024 * kernel_s * 2.4e9 / 4
+        valu = {"issue_units_per_cell": units, "busy_frac": round(issued / capacity, 4),
+                "roof_gcells": round(39.3e12 / units / 1e9, 3)}
     line = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -278,6 +303,7 @@ def main():
             "kernel": "k_step_march",
             "kernel_ms": round(kernel_s * 1e3, 5),
             "bytes_per_launch": bytes_alg,
+            "valu": valu,
         },
         "cpu_baseline": None,
         "adjoint": None,

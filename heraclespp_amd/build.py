@@ -24,7 +24,7 @@ INCLUDE = os.path.join(ROOT, "include")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhclpp_hip.so")
 
-SOURCES = ["ref_kernels.hip", "step_fused.hip", "adjoint.hip", "abi.cpp"]
+SOURCES = ["ref_kernels.hip", "step_fused.hip", "step_fused_fx.hip", "adjoint.hip", "abi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

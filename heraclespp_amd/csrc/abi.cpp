@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -524,6 +525,66 @@ int hclpp_fill_ghosts(hclpp_ctx* c, double* rho, double* rhou, double* E, double
     return HCLPP_OK;
 }
 
+// One fused step (hclpp_step, hclpp_step_f32 and their passive-scalar forms): S is the storage
+// type.  fx = NULL for the scalar-free forms, which refuse contexts with nfx > 0.
+extern "C++" {
+template <class S>
+static int fused_step(hclpp_ctx* c, const S* rho, const S* rhou, const S* E, const S* fx, S* rho_new, S* rhou_new, S* E_new, S* fx_new)
+{
+    constexpr bool f32 = std::is_same_v<S, float>;
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    int const nfx = c->cfg.nfx;
+    bool const with_fx = fx || fx_new;
+    if (!with_fx && nfx > 0)
+        return fail(HCLPP_ERR_UNSUPPORTED, f32 ? "nfx > 0: use hclpp_step_fx_f32" : "nfx > 0: use hclpp_step_fx (or the operator entry points)");
+    if (with_fx && nfx > 1)
+        return fail(HCLPP_ERR_UNSUPPORTED, "the fused step carries at most one passive scalar; use the operator entry points");
+    if (f32 && c->cfg.precision != HCLPP_FP32_STORAGE) return fail(HCLPP_ERR_ARG, "the context was not created for FP32 storage");
+    if (!f32 && c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "hclpp_step takes FP64 storage");
+    if (c->cfg.ng < 2) return fail(HCLPP_ERR_ARG, "the fused step needs ng >= 2 (two-cell reconstruction stencil)");
+    if (!rho || !rhou || !E || !rho_new || !rhou_new || !E_new) return fail(HCLPP_ERR_ARG, "null array");
+    if (nfx > 0 && (!fx || !fx_new)) return fail(HCLPP_ERR_ARG, "null passive scalar array");
+    if (rho == rho_new || E == E_new || rhou == rhou_new || (nfx > 0 && fx == fx_new))
+        return fail(HCLPP_ERR_ARG, "input and output arrays must differ");
+    StepArraysT<S> A;
+    A.rho = rho;
+    A.rhou = rhou;
+    A.E = E;
+    A.rho_new = rho_new;
+    A.rhou_new = rhou_new;
+    A.E_new = E_new;
+    A.fx = nfx > 0 ? fx : nullptr;
+    A.fx_new = nfx > 0 ? fx_new : nullptr;
+    A.nfx = nfx;
+    A.partials = c->partials;
+    A.ghost_partials = c->ghost_partials;
+    A.clock = c->clock;
+    int const nd = c->cfg.ndim;
+    auto const& eos = [&]() -> const auto& {
+        if constexpr (f32) return c->eos32;
+        else return c->eos;
+    }();
+    if (c->cfg.gravity == HCLPP_GRAV_INTERNAL_MASS) {  // main.cpp:309, :528: gravity from the state the step reads
+        launch_internal_mass_gravity(nd, c->cfg.geom, c->G, c->cfg.central_mass, rho, c->shells, c->gtab, clock_gate(c), c->stream);
+        HCLPP_LAUNCHED("internal_mass_gravity");
+    }
+    bool const timed = c->prof_count < c->prof_max;
+    if (timed) HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count], c->stream));
+    if (nfx > 0) launch_fused_step_fx(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, eos, A, c->stream);
+    else launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, eos, A, c->stream);
+    HCLPP_LAUNCHED("fused step");
+    if (timed) {
+        HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count + 1], c->stream));
+        c->prof_count += 1;
+    }
+    launch_ghosts_rate(nd, c->cfg.geom, c->G, eos, c->bc, c->shell, A, c->stream);
+    HCLPP_LAUNCHED("ghost fill");
+    launch_clock_advance(nd, c->clock, c->partials, c->fused_nb, c->ghost_partials, c->ghost_cap, c->stream);
+    HCLPP_LAUNCHED("clock advance");
+    return HCLPP_OK;
+}
+}  // extern "C++"
+
 static int clock_reset(hclpp_ctx* c, double t0, double t_end, double cfl, int max_iter)
 {
     if (!(cfl > 0) || max_iter < 1) return fail(HCLPP_ERR_ARG, "cfl must be > 0 and max_iter >= 1");
@@ -560,40 +621,13 @@ int hclpp_clock_start_f32(hclpp_ctx* c, double t0, double t_end, double cfl, int
 
 int hclpp_step_f32(hclpp_ctx* c, const float* rho, const float* rhou, const float* E, float* rho_new, float* rhou_new, float* E_new)
 {
-    if (!c) return fail(HCLPP_ERR_ARG, "null context");
-    if (c->cfg.nfx > 0) return fail(HCLPP_ERR_UNSUPPORTED, "the fused step has no passive scalars; use the operator entry points");
-    if (c->cfg.precision != HCLPP_FP32_STORAGE) return fail(HCLPP_ERR_ARG, "the context was not created for FP32 storage");
-    if (c->cfg.ng < 2) return fail(HCLPP_ERR_ARG, "the fused step needs ng >= 2 (two-cell reconstruction stencil)");
-    if (!rho || !rhou || !E || !rho_new || !rhou_new || !E_new) return fail(HCLPP_ERR_ARG, "null array");
-    if (rho == rho_new || E == E_new || rhou == rhou_new) return fail(HCLPP_ERR_ARG, "input and output arrays must differ");
-    StepArraysF A;
-    A.rho = rho;
-    A.rhou = rhou;
-    A.E = E;
-    A.rho_new = rho_new;
-    A.rhou_new = rhou_new;
-    A.E_new = E_new;
-    A.partials = c->partials;
-    A.ghost_partials = c->ghost_partials;
-    A.clock = c->clock;
-    int const nd = c->cfg.ndim;
-    if (c->cfg.gravity == HCLPP_GRAV_INTERNAL_MASS) {
-        launch_internal_mass_gravity(nd, c->cfg.geom, c->G, c->cfg.central_mass, rho, c->shells, c->gtab, clock_gate(c), c->stream);
-        HCLPP_LAUNCHED("internal_mass_gravity");
-    }
-    bool const timed = c->prof_count < c->prof_max;
-    if (timed) HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count], c->stream));
-    launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->eos32, A, c->stream);
-    HCLPP_LAUNCHED("fused step (fp32)");
-    if (timed) {
-        HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count + 1], c->stream));
-        c->prof_count += 1;
-    }
-    launch_ghosts_rate(nd, c->cfg.geom, c->G, c->eos32, c->bc, c->shell, A, c->stream);
-    HCLPP_LAUNCHED("ghost fill (fp32)");
-    launch_clock_advance(nd, c->clock, c->partials, c->fused_nb, c->ghost_partials, c->ghost_cap, c->stream);
-    HCLPP_LAUNCHED("clock advance");
-    return HCLPP_OK;
+    return fused_step<float>(c, rho, rhou, E, nullptr, rho_new, rhou_new, E_new, nullptr);
+}
+
+int hclpp_step_fx_f32(hclpp_ctx* c, const float* rho, const float* rhou, const float* E, const float* fx, float* rho_new, float* rhou_new,
+                      float* E_new, float* fx_new)
+{
+    return fused_step<float>(c, rho, rhou, E, fx, rho_new, rhou_new, E_new, fx_new);
 }
 
 int hclpp_cast_f64_to_f32(hclpp_ctx* c, const double* in, float* out, long long n)
@@ -653,40 +687,13 @@ int hclpp_clock_read(hclpp_ctx* c, hclpp_clock* out)
 
 int hclpp_step(hclpp_ctx* c, const double* rho, const double* rhou, const double* E, double* rho_new, double* rhou_new, double* E_new)
 {
-    if (!c) return fail(HCLPP_ERR_ARG, "null context");
-    if (c->cfg.nfx > 0) return fail(HCLPP_ERR_UNSUPPORTED, "the fused step has no passive scalars; use the operator entry points");
-    if (c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "hclpp_step takes FP64 storage");
-    if (c->cfg.ng < 2) return fail(HCLPP_ERR_ARG, "the fused step needs ng >= 2 (two-cell reconstruction stencil)");
-    if (!rho || !rhou || !E || !rho_new || !rhou_new || !E_new) return fail(HCLPP_ERR_ARG, "null array");
-    if (rho == rho_new || E == E_new || rhou == rhou_new) return fail(HCLPP_ERR_ARG, "input and output arrays must differ");
-    StepArrays A;
-    A.rho = rho;
-    A.rhou = rhou;
-    A.E = E;
-    A.rho_new = rho_new;
-    A.rhou_new = rhou_new;
-    A.E_new = E_new;
-    A.partials = c->partials;
-    A.ghost_partials = c->ghost_partials;
-    A.clock = c->clock;
-    int const nd = c->cfg.ndim;
-    if (c->cfg.gravity == HCLPP_GRAV_INTERNAL_MASS) {  // main.cpp:309, :528: gravity from the state the step reads
-        launch_internal_mass_gravity(nd, c->cfg.geom, c->G, c->cfg.central_mass, rho, c->shells, c->gtab, clock_gate(c), c->stream);
-        HCLPP_LAUNCHED("internal_mass_gravity");
-    }
-    bool const timed = c->prof_count < c->prof_max;
-    if (timed) HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count], c->stream));
-    launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->eos, A, c->stream);
-    HCLPP_LAUNCHED("fused step");
-    if (timed) {
-        HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count + 1], c->stream));
-        c->prof_count += 1;
-    }
-    launch_ghosts_rate(nd, c->cfg.geom, c->G, c->eos, c->bc, c->shell, A, c->stream);
-    HCLPP_LAUNCHED("ghost fill");
-    launch_clock_advance(nd, c->clock, c->partials, c->fused_nb, c->ghost_partials, c->ghost_cap, c->stream);
-    HCLPP_LAUNCHED("clock advance");
-    return HCLPP_OK;
+    return fused_step<double>(c, rho, rhou, E, nullptr, rho_new, rhou_new, E_new, nullptr);
+}
+
+int hclpp_step_fx(hclpp_ctx* c, const double* rho, const double* rhou, const double* E, const double* fx, double* rho_new, double* rhou_new,
+                  double* E_new, double* fx_new)
+{
+    return fused_step<double>(c, rho, rhou, E, fx, rho_new, rhou_new, E_new, fx_new);
 }
 
 int hclpp_step_dt(hclpp_ctx* c, double dt, const double* rho, const double* rhou, const double* E, double* rho_new, double* rhou_new,

@@ -32,6 +32,9 @@ struct StepArraysT {
     S* __restrict__ rho_new;
     S* __restrict__ rhou_new;
     S* __restrict__ E_new;
+    const S* __restrict__ fx;  // passive scalars (mass fractions, [f][cell]); unused when nfx = 0
+    S* __restrict__ fx_new;
+    int nfx;
     double* __restrict__ partials;
     double* __restrict__ ghost_partials;
     DevClock* clock;
@@ -72,6 +75,9 @@ struct MarchShape<3> {  // 32 x 8 columns (30 x 6 outputs + halo) marching along
 int64_t fused_blocks(const GridDev& G, int nd, FusedLayout* layout);
 void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, hipStream_t s);
 void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGasMixed& eos, const StepArraysF& A, hipStream_t s);
+// the fused step with one passive scalar (step_fused_fx.hip); A.fx and A.fx_new are set
+void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, hipStream_t s);
+void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGasMixed& eos, const StepArraysF& A, hipStream_t s);
 GhostShell make_ghost_shell(const GridDev& G, int nd);
 int64_t ghost_blocks(const GhostShell& B);
 void launch_ghosts_rate(int nd, int geom, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const GhostShell& B, const StepArrays& A,

@@ -1,0 +1,775 @@
+// march.hpp -- the fused MUSCL-Hancock + Riemann + Godunov step kernel (k_step_march) and
+// the per-cell helpers it shares with the ghost-fill and CFL kernels.
+//
+// Design (gfx950):
+//  * The grid is cut into in-plane tiles (x in 2D; x-y in 3D) and segments
+//    along the last active dim (the "march" dim).  A workgroup owns one tile
+//    plus a one-cell halo ring ("Hancock region", one thread per column) and
+//    marches through its segment one plane at a time.
+//  * Per plane h: each column loads the conservative state at h+1, converts
+//    to primitives, publishes plane h's primitives (plus a two-cell ring) in
+//    LDS, reconstructs all faces (limited slopes along every dim: in-plane
+//    neighbours from LDS, march neighbours from registers), converts the face
+//    states to conservative form and applies the Hancock half step -- the
+//    reference's face_reconstruction / conv_prim_to_cons / Hancock kernels for
+//    one cell, in registers.
+//  * In-plane faces: each column publishes its right face state in LDS, the
+//    right neighbour solves the Riemann problem once per face, fluxes are
+//    exchanged back through LDS.  March-dim faces: the right face state of
+//    plane h-1 is kept in registers and paired with plane h's left state.
+//  * The Godunov update of a cell accumulates dims in the reference order
+//    (flux, geometric sources, gravity per dim, march dim last), so with the
+//    FP64 policy every conserved value is bit-identical to the reference's
+//    unfused sequence.
+//  * Face states never touch HBM: per cell-update the kernel reads the
+//    conservative state once (plus halo re-reads served mostly by L2) and
+//    writes it once.  The epilogue reduces (in FP64) the CFL rate of the new
+//    state, the minimum internal energy of the consumed state and the
+//    conserved totals; per-workgroup partials are folded in a fixed order.
+//  * The kernels are generic over the precision policy (physics.hpp):
+//    PerfectGas streams FP64; PerfectGasMixed streams FP32 (half the bytes)
+//    with FP64 pressure recovery and FP64 reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "fused.hpp"
+#include "launch.hpp"
+#include "physics.hpp"
+#include "reduce.hpp"
+
+namespace hclpp {
+
+template <int ND>
+__device__ __forceinline__ int64_t cell_at(const GridDev& G, int a, int b, int m)
+{
+    if constexpr (ND == 1) return (int64_t)m;
+    else if constexpr (ND == 2) return (int64_t)a + (int64_t)G.nwg[0] * (int64_t)m;
+    else return flat(G, a, b, m);
+}
+
+template <int ND>
+__device__ __forceinline__ void ijk_of(int a, int b, int m, int& i, int& j, int& k)
+{
+    if constexpr (ND == 1) { i = m; j = 0; k = 0; }
+    else if constexpr (ND == 2) { i = a; j = m; k = 0; }
+    else { i = a; j = b; k = m; }
+}
+
+template <int ND, class S>
+__device__ __forceinline__ State<ND, S> load_cons(const StepArraysT<S>& A, const GridDev& G, int64_t id)
+{
+    State<ND, S> c;
+    c[0] = A.rho[id];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) c[1 + d] = A.rhou[id + G.plane * d];
+    c[ND + 1] = A.E[id];
+    return c;
+}
+
+template <int ND, class S>
+__device__ __forceinline__ void store_cons(const StepArraysT<S>& A, const GridDev& G, int64_t id, const State<ND, S>& c)
+{
+    A.rho_new[id] = c[0];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) A.rhou_new[id + G.plane * d] = c[1 + d];
+    A.E_new[id] = c[ND + 1];
+}
+
+// CFL rate of a conservative state at (i,j,k) (time_step.hpp:61-76), in FP64.  With the
+// FP64 policy this is the reference's arithmetic (conv_cons_to_prim, then the rate).
+template <int ND, int GEOM, class Eos>
+__device__ __forceinline__ double state_rate(const GridDev& G, const Eos& eos, int i, int j, int k, const State<ND, typename Eos::real>& c)
+{
+    if constexpr (std::is_same_v<Eos, PerfectGas>) {
+        State<ND> const p = cons_to_prim(c, eos);
+        return cfl_rate<ND, GEOM>(G, i, j, k, &p.q[1], eos.sound(p[0], p[ND + 1]));
+    } else {
+        double const rho = c[0];
+        double u[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) u[d] = (double)c[1 + d] / rho;
+        double const P = eos.gamma_m1 * eos.internal_energy_fp64(c);
+        return cfl_rate<ND, GEOM>(G, i, j, k, u, sqrt(eos.gamma_d * P / rho));
+    }
+}
+
+// Internal energy E - |m|^2 / 2 rho of a conservative state in FP64 (internal_energy.cpp:25-40).
+template <int ND, class Eos>
+__device__ __forceinline__ double state_eint(const Eos& eos, const State<ND, typename Eos::real>& c)
+{
+    if constexpr (std::is_same_v<Eos, PerfectGas>) return internal_energy(c);
+    else return eos.internal_energy_fp64(c);
+}
+
+// limited_linear_reconstruction.hpp:53-62 for one variable along one dim
+template <int LIM, class Eos, class T>
+__device__ __forceinline__ void reconstruct(T vm, T v, T vp, T dlm, T dl, T dlp, T& lo, T& hi)
+{
+    T const slope = limited_slope_t<LIM, Eos, T>(Eos::div(vp - v, (dl + dlp) * T(0.5)), Eos::div(v - vm, (dlm + dl) * T(0.5)));
+    lo = v - ((dl * T(0.5)) * slope);
+    hi = v + ((dl * T(0.5)) * slope);
+}
+
+// Spherical sources of one dim for the cell's own face states (godunov_scheme.hpp:165-256),
+// applied to the accumulator in the reference order.
+template <class Eos, int ND, int GEOM>
+__device__ __forceinline__ void godunov_sources(const GridDev& G, const Eos& eos, int dim, int i, int j, typename Eos::real dtodv,
+                                                typename Eos::real dS, typename Eos::real dSp, const State<ND, typename Eos::real>& vL,
+                                                const State<ND, typename Eos::real>& vR, State<ND, typename Eos::real>& un)
+{
+    using T = typename Eos::real;
+    if constexpr (GEOM == kSpherical) {
+        if (dim > 1) return;
+        State<ND, T> const pL = cons_to_prim(vL, eos);
+        State<ND, T> const pR = cons_to_prim(vR, eos);
+        T const x0 = T(G.x[0][i]), x0p = T(G.x[0][i + 1]);
+        if (dim == 0) {
+            un[1] += src_pressure<Eos>(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+            if constexpr (ND >= 2) {
+                un[1] += src_centrifugal<Eos>(dtodv, pL[0], pR[0], pL[2], pR[2], dS, dSp);
+                un[2] -= src_coriolis_r<Eos>(dtodv, x0, x0p, pL[0], pR[0], pL[1], pR[1], pL[2], pR[2], dS, dSp);
+            }
+            if constexpr (ND == 3) {
+                un[1] += src_centrifugal<Eos>(dtodv, pL[0], pR[0], pL[3], pR[3], dS, dSp);
+                un[3] -= src_coriolis_r<Eos>(dtodv, x0, x0p, pL[0], pR[0], pL[1], pR[1], pL[3], pR[3], dS, dSp);
+            }
+        } else {
+            un[2] += src_pressure<Eos>(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+            if constexpr (ND == 3) {
+                un[2] += src_centrifugal_th<Eos>(dtodv, T(G.cot_mid[j]), pL[0], pR[0], pL[3], pR[3], dS, dSp);
+                un[3] -= src_coriolis_th<Eos>(dtodv, T(G.sin_node[j]), T(G.sin_node[j + 1]), pL[0], pR[0], pL[2], pR[2], pL[3], pR[3], dS,
+                                              dSp);
+            }
+        }
+    }
+}
+
+// Passive scalars of one cell or face (mass fractions, or partial densities during an update);
+// a one-slot placeholder when the step has none.
+template <int NFX, class T>
+struct Frac {
+    T f[NFX > 0 ? NFX : 1];
+};
+
+template <int NFX, class T>
+__device__ __forceinline__ Frac<NFX, T> zero_frac()
+{
+    Frac<NFX, T> x;
+#pragma unroll
+    for (int f = 0; f < (NFX > 0 ? NFX : 1); ++f) x.f[f] = 0;
+    return x;
+}
+
+// MUSCL-Hancock predictor of one cell (hydro_reconstruction.hpp:99-128 and
+// extrapolation_reconstruction.hpp:90-337): nb[dim][side] are the primitive
+// neighbours (side 0 = minus, 1 = plus) along each dim; x and xnb the passive
+// scalars of the cell and of the same neighbours, xf their face values after the
+// half step (mass fractions, as fx_rec leaves the predictor).
+template <class Eos, int ND, int GEOM, int LIM, int NFX>
+__device__ __forceinline__ void predict_faces(const GridDev& G, const Eos& eos, typename Eos::real dth, double dth_d, int i, int j, int k,
+                                              const State<ND, typename Eos::real>& p, const State<ND, typename Eos::real> (&nb)[ND][2],
+                                              State<ND, typename Eos::real> (&fs)[2][ND], const Frac<NFX, typename Eos::real>& x,
+                                              const Frac<NFX, typename Eos::real> (&xnb)[ND][2], Frac<NFX, typename Eos::real> (&xf)[2][ND])
+{
+    using T = typename Eos::real;
+    constexpr int NV = ND + 2;
+    State<ND, T> fp[2][ND];
+    Frac<NFX, T> xo[2][ND];  // reconstructed mass fractions, kept for the scalar fluxes (extrapolation_reconstruction.hpp:87-88)
+    int const idx[3] = {i, j, k};
+#pragma unroll
+    for (int dim = 0; dim < ND; ++dim) {
+        T const dl = T(G.dx[dim][idx[dim]]);
+        T const dlm = T(G.dx[dim][idx[dim] - 1]);
+        T const dlp = T(G.dx[dim][idx[dim] + 1]);
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            reconstruct<LIM, Eos>(nb[dim][0][v], p[v], nb[dim][1][v], dlm, dl, dlp, fp[0][dim][v], fp[1][dim][v]);
+        if constexpr (NFX > 0) {
+#pragma unroll
+            for (int f = 0; f < NFX; ++f)
+                reconstruct<LIM, Eos>(xnb[dim][0].f[f], x.f[f], xnb[dim][1].f[f], dlm, dl, dlp, xo[0][dim].f[f], xo[1][dim].f[f]);
+        }
+    }
+#pragma unroll
+    for (int dim = 0; dim < ND; ++dim)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) fs[s][dim] = prim_to_cons(fp[s][dim], eos);
+    T rho_old[2][ND], mom_old[2][ND][ND];
+#pragma unroll
+    for (int dim = 0; dim < ND; ++dim)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            rho_old[s][dim] = fs[s][dim][0];
+#pragma unroll
+            for (int v = 0; v < ND; ++v) mom_old[s][dim][v] = fs[s][dim][1 + v];
+        }
+    if constexpr (NFX > 0) {  // extrapolation_reconstruction.hpp:105-108: partial densities of the faces
+#pragma unroll
+        for (int dim = 0; dim < ND; ++dim)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int f = 0; f < NFX; ++f) xf[s][dim].f[f] = xo[s][dim].f[f] * rho_old[s][dim];
+    }
+    T const dtodv = T(dth_d / cell_volume<ND, GEOM>(G, i, j, k));
+    T const x0 = T(G.x[0][i]), x0p = T(G.x[0][i + 1]);
+#pragma unroll
+    for (int dim = 0; dim < ND; ++dim) {
+        State<ND, T> const& pL = fp[0][dim];
+        State<ND, T> const& pR = fp[1][dim];
+        State<ND, T> const fL = flux_of_prim(pL, dim, eos);
+        State<ND, T> const fR = flux_of_prim(pR, dim, eos);
+        T const dS = T(face_area<ND, GEOM>(G, dim, i, j, k));
+        T const dSp = T(face_area<ND, GEOM>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2)));
+#pragma unroll
+        for (int pos = 0; pos < ND; ++pos)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) fs[s][pos][v] += dtodv * ((fL[v] * dS) - (fR[v] * dSp));
+        if constexpr (GEOM == kSpherical) {
+#pragma unroll
+            for (int pos = 0; pos < ND; ++pos)
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    if (dim == 0) {
+                        fs[s][pos][1] += src_pressure<Eos>(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+                        if constexpr (ND >= 2) {
+                            fs[s][pos][1] += src_centrifugal<Eos>(dtodv, pL[0], pR[0], pL[2], pR[2], dS, dSp);
+                            fs[s][pos][2] -= src_coriolis_r<Eos>(dtodv, x0, x0p, pL[0], pR[0], pL[1], pR[1], pL[2], pR[2], dS, dSp);
+                        }
+                        if constexpr (ND == 3) {
+                            fs[s][pos][1] += src_centrifugal<Eos>(dtodv, pL[0], pR[0], pL[3], pR[3], dS, dSp);
+                            fs[s][pos][3] -= src_coriolis_r<Eos>(dtodv, x0, x0p, pL[0], pR[0], pL[1], pR[1], pL[3], pR[3], dS, dSp);
+                        }
+                    }
+                    if (dim == 1) {
+                        fs[s][pos][2] += src_pressure<Eos>(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+                        if constexpr (ND == 3) {
+                            fs[s][pos][2] += src_centrifugal_th<Eos>(dtodv, T(G.cot_mid[j]), pL[0], pR[0], pL[3], pR[3], dS, dSp);
+                            fs[s][pos][3] -= src_coriolis_th<Eos>(dtodv, T(G.sin_node[j]), T(G.sin_node[j + 1]), pL[0], pR[0], pL[2], pR[2],
+                                                                  pL[3], pR[3], dS, dSp);
+                        }
+                    }
+                }
+        }
+        T const g = T(gravity(G, i, dim));
+#pragma unroll
+        for (int pos = 0; pos < ND; ++pos)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                fs[s][pos][1 + dim] += dth * g * rho_old[s][pos];
+                fs[s][pos][ND + 1] += dth * g * mom_old[s][pos][dim];
+            }
+        if constexpr (NFX > 0) {  // extrapolation_reconstruction.hpp:315-323
+#pragma unroll
+            for (int f = 0; f < NFX; ++f) {
+                T const ffL = xo[0][dim].f[f] * fL[0];
+                T const ffR = xo[1][dim].f[f] * fR[0];
+#pragma unroll
+                for (int pos = 0; pos < ND; ++pos)
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) xf[s][pos].f[f] += dtodv * ((ffL * dS) - (ffR * dSp));
+            }
+        }
+    }
+    if constexpr (NFX > 0) {  // passive_scalar_extrapolation (extrapolation_reconstruction.hpp:327-337)
+#pragma unroll
+        for (int dim = 0; dim < ND; ++dim)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int f = 0; f < NFX; ++f) xf[s][dim].f[f] = Eos::div(xf[s][dim].f[f], fs[s][dim][0]);
+    }
+}
+
+// Safe stand-in for cells outside the array (never feeds a valid output).
+template <int ND, class T>
+__device__ __forceinline__ State<ND, T> dummy_state()
+{
+    State<ND, T> p;
+    p[0] = 1;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) p[1 + d] = 0;
+    p[ND + 1] = 1;
+    return p;
+}
+
+// Minimum waves per SIMD the register allocator must allow.  Measured on MI355X: the 3D
+// FP32-storage kernel gains 23 % at 3 waves (168 VGPRs, 20 B of spills per lane); the FP64
+// kernels lose when forced above their natural 1 (3D) or 2 (2D) waves, because they spill.
+template <class Eos, int ND>
+constexpr int march_min_waves() { return (!std::is_same_v<Eos, PerfectGas> && ND == 3) ? 3 : 1; }
+
+// Load the NFX passive scalars (mass fractions, layout [f][cell]) of one cell.
+template <int NFX, class S>
+__device__ __forceinline__ Frac<NFX, S> load_frac(const StepArraysT<S>& A, const GridDev& G, int64_t id)
+{
+    Frac<NFX, S> x;
+#pragma unroll
+    for (int f = 0; f < NFX; ++f) x.f[f] = A.fx[id + G.plane * f];
+    return x;
+}
+
+// NFX: passive scalars carried through the step (0, or 1 for the kernels of step_fused_fx.hip).
+template <class Eos, int ND, int GEOM, int RS, int LIM, int NFX, int HX, int HY, int NT>
+__global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march(GridDev G, Eos eos, StepArraysT<typename Eos::real> A, FusedLayout L)
+{
+    using T = typename Eos::real;
+    using X = Frac<NFX, T>;
+    constexpr int NV = ND + 2;
+    constexpr int NL = NV + NFX;             // values per column in the LDS planes
+    constexpr int NP = ND - 1;               // in-plane dims
+    constexpr int MD = ND - 1;               // march dim
+    constexpr int PX = (ND >= 2) ? HX + 2 : 1;
+    constexpr int PY = (ND == 3) ? HY + 2 : 1;
+    constexpr int TX = (ND >= 2) ? HX - 2 : 1;
+    constexpr int TY = (ND == 3) ? HY - 2 : 1;
+    constexpr int NXCH = NP > 0 ? NP : 1;
+    // Three barriers per plane: the primitive plane is double-buffered by plane parity, and
+    // face states and fluxes have buffers of their own, so no write waits for the readers of
+    // the previous plane's buffer beyond the barrier that already separates them.
+    __shared__ T s_prim[2][NL][PY][PX];
+    __shared__ T s_st[NXCH][NL][HY][HX];
+    __shared__ T s_fl[NXCH][NL][HY][HX];
+    __shared__ double s_red[NT / 64];
+
+    DevClock const* clk = A.clock;
+    if (clk->pub.done || clk->pub.status) return;
+    double const dt_d = clk->pub.dt;
+    double const dth_d = dt_d / 2;
+    T const dt = T(dt_d);
+    T const dth = T(dth_d);
+
+    int const tid = threadIdx.x;
+    // XCD-aware block order: blocks that share an XCD (b % 8) get consecutive tiles.
+    int const nb = gridDim.x;
+    int const b = blockIdx.x;
+    int const q = nb / 8, rem = nb % 8, x8 = b % 8;
+    int const lb = x8 * q + (x8 < rem ? x8 : rem) + b / 8;
+
+    int hx = 0, hy = 0, seg;
+    int ox = 0, oy = 0;
+    if constexpr (ND == 1) {
+        seg = lb * NT + tid;
+    } else {
+        hx = tid % HX;
+        hy = tid / HX;
+        int const tx = lb % L.ntile[0];
+        int const rest = lb / L.ntile[0];
+        int const ty = rest % L.ntile[1];
+        seg = rest / L.ntile[1];
+        ox = G.ngh[0] + tx * TX;
+        oy = (ND == 3) ? G.ngh[1] + ty * TY : 0;
+    }
+    bool const in_block = (ND == 1) || tid < HX * HY;
+    int const a = ox - 1 + hx;                  // column coords (ghost-inclusive)
+    int const bb = (ND == 3) ? oy - 1 + hy : 0;
+    int const ngm = G.ngh[MD], nm = G.n[MD];
+    int const zs = ngm + seg * L.seglen;
+    int const ze = min(zs + L.seglen, ngm + nm);
+
+    bool col_in_array = in_block;               // column exists in memory
+    bool hancock_ok = in_block;                 // column needs a predictor
+    bool out_col = in_block;                    // column holds output cells
+    if constexpr (ND >= 2) {
+        col_in_array = col_in_array && a < G.nwg[0];
+        hancock_ok = hancock_ok && a <= G.ngh[0] + G.n[0];
+        out_col = out_col && hx >= 1 && hx <= TX && a < G.ngh[0] + G.n[0];
+    }
+    if constexpr (ND == 3) {
+        col_in_array = col_in_array && bb < G.nwg[1];
+        hancock_ok = hancock_ok && bb <= G.ngh[1] + G.n[1];
+        out_col = out_col && hy >= 1 && hy <= TY && bb < G.ngh[1] + G.n[1];
+    }
+    bool const seg_ok = zs < ngm + nm;
+    if constexpr (ND == 1) {
+        col_in_array = seg_ok;
+        hancock_ok = seg_ok;
+        out_col = seg_ok;
+    }
+
+    // ring cells of the primitive plane (in-plane distance 2 from the output tile)
+    constexpr int NRING = (ND == 3) ? 2 * (PX - 2) + 2 * (PY - 2) : ((ND == 2) ? 2 : 0);
+
+    double r_inv = reduce_identity<kMax>();
+    double r_emin = reduce_identity<kMin>();
+    double r_sum[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) r_sum[v] = 0;
+
+    auto prim_at = [&](int ca, int cb, int m) -> State<ND, T> {
+        bool ok = m >= 0 && m < G.nwg[MD];
+        if constexpr (ND >= 2) ok = ok && ca >= 0 && ca < G.nwg[0];
+        if constexpr (ND == 3) ok = ok && cb >= 0 && cb < G.nwg[1];
+        if (!ok) return dummy_state<ND, T>();
+        return cons_to_prim(load_cons<ND>(A, G, cell_at<ND>(G, ca, cb, m)), eos);
+    };
+    auto frac_at = [&](int ca, int cb, int m) -> X {
+        bool ok = m >= 0 && m < G.nwg[MD];
+        if constexpr (ND >= 2) ok = ok && ca >= 0 && ca < G.nwg[0];
+        if constexpr (ND == 3) ok = ok && cb >= 0 && cb < G.nwg[1];
+        if (!ok) return zero_frac<NFX, T>();
+        return load_frac<NFX>(A, G, cell_at<ND>(G, ca, cb, m));
+    };
+
+    // ring cell of this thread (at most one: NRING <= NT), fixed for the whole march
+    static_assert(NRING <= NT, "one ring cell per thread");
+    bool const has_ring = tid < NRING;
+    int ring_px = 0, ring_py = 0;
+    if constexpr (ND == 2) {
+        ring_px = tid == 0 ? 0 : PX - 1;
+    } else if constexpr (ND == 3) {
+        int const r = tid;
+        if (r < PX - 2) { ring_px = 1 + r; ring_py = 0; }
+        else if (r < 2 * (PX - 2)) { ring_px = 1 + r - (PX - 2); ring_py = PY - 1; }
+        else if (r < 2 * (PX - 2) + (PY - 2)) { ring_px = 0; ring_py = 1 + r - 2 * (PX - 2); }
+        else { ring_px = PX - 1; ring_py = 1 + r - 2 * (PX - 2) - (PY - 2); }
+    }
+    int const ring_a = ox - 2 + ring_px, ring_b = (ND == 3) ? oy - 2 + ring_py : 0;
+    bool ring_in = has_ring;
+    if constexpr (ND >= 2) ring_in = ring_in && ring_a >= 0 && ring_a < G.nwg[0];
+    if constexpr (ND == 3) ring_in = ring_in && ring_b >= 0 && ring_b < G.nwg[1];
+
+    // pipeline registers.  Global loads run one plane ahead of their use (own column: plane h+2
+    // while plane h is processed; ring cell: plane h+1), so their latency hides behind a plane
+    // of arithmetic instead of stalling the workgroup at the LDS barrier.
+    State<ND, T> pm, p0, pp, c0, cp, cnext, rnext;
+    State<ND, T> fsR_prev, Fm_prev, part_prev;
+    T old_rho_prev = 0, old_mom_prev = 0, src_prev = 0;
+    // passive scalars: mass fractions of planes h-1, h, h+1 and the prefetch; the ring cell's
+    // prefetch; the right march face of plane h-1, its partial-density update and the
+    // upwinded scalar flux of its left march face (godunov_scheme.hpp:263-290)
+    X xm = zero_frac<NFX, T>(), x0 = xm, xp = xm, xnext = xm, rxnext = xm;
+    X xfR_prev = xm, xpart_prev = xm, ffm_prev = xm;
+    int const zfirst = seg_ok ? zs - 1 : 0;
+    int const zlast = seg_ok ? ze : -1;
+    bool rnext_ok = false;
+    if (col_in_array) {
+        pm = prim_at(a, bb, zfirst - 1);
+        c0 = load_cons<ND>(A, G, cell_at<ND>(G, a, bb, zfirst));
+        p0 = cons_to_prim(c0, eos);
+        if (zfirst <= zlast) cnext = load_cons<ND>(A, G, cell_at<ND>(G, a, bb, zfirst + 1));
+        if constexpr (NFX > 0) {
+            xm = frac_at(a, bb, zfirst - 1);
+            x0 = load_frac<NFX>(A, G, cell_at<ND>(G, a, bb, zfirst));
+            if (zfirst <= zlast) xnext = load_frac<NFX>(A, G, cell_at<ND>(G, a, bb, zfirst + 1));
+        }
+    } else {
+        pm = dummy_state<ND, T>();
+        p0 = dummy_state<ND, T>();
+        c0 = dummy_state<ND, T>();
+    }
+    if constexpr (ND >= 2) {
+        rnext_ok = ring_in && zfirst >= 0 && zfirst < G.nwg[MD] && zfirst <= zlast;
+        if (rnext_ok) {
+            rnext = load_cons<ND>(A, G, cell_at<ND>(G, ring_a, ring_b, zfirst));
+            if constexpr (NFX > 0) rxnext = load_frac<NFX>(A, G, cell_at<ND>(G, ring_a, ring_b, zfirst));
+        }
+    }
+
+    for (int h = zfirst; h <= zlast; ++h) {
+        int const pb = h & 1;
+        // ---- plane h+1 of this column (prefetched), and the prefetch of plane h+2
+        if (col_in_array) {
+            cp = cnext;
+            xp = xnext;
+            if (h + 1 <= zlast) {
+                cnext = load_cons<ND>(A, G, cell_at<ND>(G, a, bb, h + 2));
+                if constexpr (NFX > 0) xnext = load_frac<NFX>(A, G, cell_at<ND>(G, a, bb, h + 2));
+            }
+            pp = cons_to_prim(cp, eos);
+        } else {
+            cp = dummy_state<ND, T>();
+            pp = dummy_state<ND, T>();
+        }
+        // ---- plane h primitives in LDS (own column + ring)
+        if constexpr (ND >= 2) {
+            State<ND, T> ring_c;
+            X const rx = rnext_ok ? rxnext : zero_frac<NFX, T>();
+            bool const ring_ok = rnext_ok;
+            if (ring_ok) ring_c = rnext;
+            rnext_ok = ring_in && h + 1 < G.nwg[MD] && h + 1 <= zlast;
+            if (rnext_ok) {
+                rnext = load_cons<ND>(A, G, cell_at<ND>(G, ring_a, ring_b, h + 1));
+                if constexpr (NFX > 0) rxnext = load_frac<NFX>(A, G, cell_at<ND>(G, ring_a, ring_b, h + 1));
+            }
+            State<ND, T> const pr = ring_ok ? cons_to_prim(ring_c, eos) : dummy_state<ND, T>();
+            // buffer h & 1 was last read at plane h - 2, before the barrier of plane h - 1
+            if (in_block) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) s_prim[pb][v][(ND == 3) ? hy + 1 : 0][hx + 1] = p0[v];
+#pragma unroll
+                for (int f = 0; f < NFX; ++f) s_prim[pb][NV + f][(ND == 3) ? hy + 1 : 0][hx + 1] = x0.f[f];
+            }
+            if (has_ring) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) s_prim[pb][v][ring_py][ring_px] = pr[v];
+#pragma unroll
+                for (int f = 0; f < NFX; ++f) s_prim[pb][NV + f][ring_py][ring_px] = rx.f[f];
+            }
+            __syncthreads();
+        }
+        int ci, cj, ck;
+        ijk_of<ND>(a, bb, h, ci, cj, ck);
+        bool const out_plane = h >= zs && h < ze;
+
+        // ---- predictor of plane h
+        State<ND, T> fs[2][ND];
+        X xf[2][ND];
+        if (hancock_ok) {
+            State<ND, T> nb3[ND][2];
+            X xnb[ND][2];
+            if constexpr (ND >= 2) {
+                int const py0 = (ND == 3) ? hy + 1 : 0;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    nb3[0][0][v] = s_prim[pb][v][py0][hx];
+                    nb3[0][1][v] = s_prim[pb][v][py0][hx + 2];
+                }
+#pragma unroll
+                for (int f = 0; f < NFX; ++f) {
+                    xnb[0][0].f[f] = s_prim[pb][NV + f][py0][hx];
+                    xnb[0][1].f[f] = s_prim[pb][NV + f][py0][hx + 2];
+                }
+                if constexpr (ND == 3) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        nb3[1][0][v] = s_prim[pb][v][hy][hx + 1];
+                        nb3[1][1][v] = s_prim[pb][v][hy + 2][hx + 1];
+                    }
+#pragma unroll
+                    for (int f = 0; f < NFX; ++f) {
+                        xnb[1][0].f[f] = s_prim[pb][NV + f][hy][hx + 1];
+                        xnb[1][1].f[f] = s_prim[pb][NV + f][hy + 2][hx + 1];
+                    }
+                }
+            }
+            nb3[MD][0] = pm;
+            nb3[MD][1] = pp;
+            xnb[MD][0] = xm;
+            xnb[MD][1] = xp;
+            predict_faces<Eos, ND, GEOM, LIM, NFX>(G, eos, dth, dth_d, ci, cj, ck, p0, nb3, fs, x0, xnb, xf);
+        } else {
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    fs[s][d] = c0;
+                    xf[s][d] = x0;
+                }
+        }
+
+        // ---- in-plane fluxes and the in-plane part of the update of plane h
+        State<ND, T> part = c0;
+        T old_rho = c0[0], old_mom = c0[1 + MD], src_m = 0;
+        X xpart;  // partial densities fx * rho (godunov_scheme.hpp:122) plus the in-plane fluxes
+#pragma unroll
+        for (int f = 0; f < NFX; ++f) xpart.f[f] = x0.f[f] * c0[0];
+        T dtodv = 0;
+        if (out_plane) {
+            if (out_col) {
+                dtodv = T(dt_d / cell_volume<ND, GEOM>(G, ci, cj, ck));
+                // min internal energy of the consumed state (internal_energy.cpp:25-40)
+                r_emin = reduce_combine<kMin>(r_emin, state_eint<ND>(eos, c0));
+            }
+            if constexpr (ND >= 2) {
+                State<ND, T> FLp[NXCH], FRp[NXCH];
+                X XLp[NXCH], XRp[NXCH];  // upwinded scalar fluxes of the left and right faces
+                // right face states of every in-plane dim; s_st was last read before the
+                // flux barrier of the previous plane
+                if (in_block) {
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) s_st[p][v][hy][hx] = fs[1][p][v];
+#pragma unroll
+                        for (int f = 0; f < NFX; ++f) s_st[p][NV + f][hy][hx] = xf[1][p].f[f];
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    bool need;
+                    if (p == 0) need = hancock_ok && hx >= 1 && (ND == 2 || (hy >= 1 && hy <= TY));
+                    else need = hancock_ok && hy >= 1 && hx >= 1 && hx <= TX;
+                    State<ND, T> F;
+                    X XF = zero_frac<NFX, T>();
+                    if (need) {
+                        State<ND, T> left;
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) left[v] = (p == 0) ? s_st[p][v][hy][hx - 1] : s_st[p][v][hy - 1][hx];
+                        F = riemann<RS>(left, fs[0][p], p, eos);
+                        // upwind face of the scalar (godunov_scheme.hpp:265-287): this cell's
+                        // left face when the mass flux is negative, else the left cell's right face
+#pragma unroll
+                        for (int f = 0; f < NFX; ++f) {
+                            T const up = F[0] < 0 ? xf[0][p].f[f] : ((p == 0) ? s_st[p][NV + f][hy][hx - 1] : s_st[p][NV + f][hy - 1][hx]);
+                            XF.f[f] = up * F[0];
+                        }
+                    } else {
+                        F = c0;
+                    }
+                    FLp[p] = F;
+                    XLp[p] = XF;
+                    // s_fl was last read before the primitive barrier of this plane
+                    if (in_block) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) s_fl[p][v][hy][hx] = F[v];
+#pragma unroll
+                        for (int f = 0; f < NFX; ++f) s_fl[p][NV + f][hy][hx] = XF.f[f];
+                    }
+                }
+                __syncthreads();
+                if (out_col) {
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) FRp[p][v] = (p == 0) ? s_fl[p][v][hy][hx + 1] : s_fl[p][v][hy + 1][hx];
+#pragma unroll
+                        for (int f = 0; f < NFX; ++f) XRp[p].f[f] = (p == 0) ? s_fl[p][NV + f][hy][hx + 1] : s_fl[p][NV + f][hy + 1][hx];
+                    }
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) {
+                        T const dS = T(face_area<ND, GEOM>(G, p, ci, cj, ck));
+                        T const dSp = T(face_area<ND, GEOM>(G, p, ci + (p == 0), cj + (p == 1), ck));
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) part[v] += dtodv * ((FLp[p][v] * dS) - (FRp[p][v] * dSp));
+                        godunov_sources<Eos, ND, GEOM>(G, eos, p, ci, cj, dtodv, dS, dSp, fs[0][p], fs[1][p], part);
+                        T const g = T(gravity(G, ci, p));
+                        part[1 + p] += dt * g * c0[0];
+                        part[ND + 1] += dt * g * c0[1 + p];
+#pragma unroll
+                        for (int f = 0; f < NFX; ++f) xpart.f[f] += dtodv * ((XLp[p].f[f] * dS) - (XRp[p].f[f] * dSp));
+                    }
+                }
+            }
+            // geometric pressure source of the march dim (1D spherical: r; 2D spherical: theta)
+            if constexpr (GEOM == kSpherical && MD <= 1) {
+                if (out_col) {
+                    State<ND, T> const pL = cons_to_prim(fs[0][MD], eos);
+                    State<ND, T> const pR = cons_to_prim(fs[1][MD], eos);
+                    T const dS = T(face_area<ND, GEOM>(G, MD, ci, cj, ck));
+                    T const dSp = T(face_area<ND, GEOM>(G, MD, ci + (MD == 0), cj + (MD == 1), ck));
+                    src_m = src_pressure<Eos>(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+                }
+            }
+        }
+
+        // ---- march-dim face h-1/2 and completion of plane h-1
+        if (h >= zs && out_col) {
+            State<ND, T> const Fm = riemann<RS>(fsR_prev, fs[0][MD], MD, eos);
+            X XFm;  // upwinded scalar flux of the face h-1/2
+#pragma unroll
+            for (int f = 0; f < NFX; ++f) XFm.f[f] = (Fm[0] < 0 ? xf[0][MD].f[f] : xfR_prev.f[f]) * Fm[0];
+            int const hm = h - 1;
+            if (hm >= zs) {
+                int pi, pj, pk;
+                ijk_of<ND>(a, bb, hm, pi, pj, pk);
+                double const vol = cell_volume<ND, GEOM>(G, pi, pj, pk);
+                T const dtodv_m = T(dt_d / vol);
+                T const dS = T(face_area<ND, GEOM>(G, MD, pi, pj, pk));
+                T const dSp = T(face_area<ND, GEOM>(G, MD, ci, cj, ck));
+                State<ND, T> un = part_prev;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) un[v] += dtodv_m * ((Fm_prev[v] * dS) - (Fm[v] * dSp));
+                if constexpr (GEOM == kSpherical && MD <= 1) un[1 + MD] += src_prev;
+                T const g = T(gravity(G, pi, MD));
+                un[1 + MD] += dt * g * old_rho_prev;
+                un[ND + 1] += dt * g * old_mom_prev;
+                store_cons<ND>(A, G, cell_at<ND>(G, a, bb, hm), un);
+                if constexpr (NFX > 0) {  // godunov_scheme.hpp:288, then passive_scalar_Godunov (:294-301)
+                    int64_t const id = cell_at<ND>(G, a, bb, hm);
+#pragma unroll
+                    for (int f = 0; f < NFX; ++f) {
+                        T const xn = xpart_prev.f[f] + dtodv_m * ((ffm_prev.f[f] * dS) - (XFm.f[f] * dSp));
+                        T const y = Eos::div(xn, un[0]);
+                        A.fx_new[id + G.plane * f] = y < T(0) ? T(0) : (y > T(1) ? T(1) : y);
+                    }
+                }
+                // epilogue: CFL rate of the new state and the conserved totals (FP64)
+                r_inv = reduce_combine<kMax>(r_inv, state_rate<ND, GEOM>(G, eos, pi, pj, pk, un));
+#pragma unroll
+                for (int v = 0; v < NV; ++v) r_sum[v] += (double)un[v] * vol;
+            }
+            Fm_prev = Fm;
+            ffm_prev = XFm;
+        }
+
+        // ---- rotate the pipeline
+        fsR_prev = fs[1][MD];
+        part_prev = part;
+        xfR_prev = xf[1][MD];
+        xpart_prev = xpart;
+        xm = x0;
+        x0 = xp;
+        old_rho_prev = old_rho;
+        old_mom_prev = old_mom;
+        src_prev = src_m;
+        pm = p0;
+        p0 = pp;
+        c0 = cp;
+    }
+
+    // ---- workgroup partials
+    r_inv = block_reduce<kMax>(r_inv, s_red, tid, NT);
+    r_emin = block_reduce<kMin>(r_emin, s_red, tid, NT);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) r_sum[v] = block_reduce<kSum>(r_sum[v], s_red, tid, NT);
+    if (tid == 0) {
+        double* out = A.partials + (int64_t)blockIdx.x * kPartialStride;
+        out[0] = r_inv;
+        out[1] = r_emin;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) out[2 + v] = r_sum[v];
+    }
+}
+
+// Launch of k_step_march over the whole interior (one plane segment per workgroup column tile).
+template <class Eos, int ND, int GEOM, int RS, int LIM, int NFX>
+static void launch_march_t(const GridDev& G, const Eos& eos, const StepArraysT<typename Eos::real>& A, hipStream_t s)
+{
+    using S = MarchShape<ND>;
+    FusedLayout L;
+    int64_t const nb = fused_blocks(G, ND, &L);
+    if (nb == 0) return;
+    hipLaunchKernelGGL((k_step_march<Eos, ND, GEOM, RS, LIM, NFX, S::HX, S::HY, S::NT>), dim3((unsigned)nb), dim3(S::NT), 0, s, G, eos, A, L);
+}
+
+// Runtime dispatch over dimension, geometry, Riemann solver and limiter.
+template <class Eos, int NFX>
+static void launch_march_any(int nd, int geom, int rs, int lim, const GridDev& G, const Eos& eos, const StepArraysT<typename Eos::real>& A,
+                             hipStream_t s)
+{
+#define HCLPP_F4(NDV, GV, RSV, LV) launch_march_t<Eos, NDV, GV, RSV, LV, NFX>(G, eos, A, s)
+#define HCLPP_FL(NDV, GV, RSV)                                     \
+    switch (lim) {                                                 \
+    case kConstant: HCLPP_F4(NDV, GV, RSV, kConstant); break;      \
+    case kVanLeer: HCLPP_F4(NDV, GV, RSV, kVanLeer); break;        \
+    case kMinmod: HCLPP_F4(NDV, GV, RSV, kMinmod); break;          \
+    default: HCLPP_F4(NDV, GV, RSV, kVanAlbada); break;            \
+    }
+#define HCLPP_FR(NDV, GV)                              \
+    switch (rs) {                                      \
+    case kHLL: HCLPP_FL(NDV, GV, kHLL); break;         \
+    case kHLLC: HCLPP_FL(NDV, GV, kHLLC); break;       \
+    default: HCLPP_FL(NDV, GV, kLowMach); break;       \
+    }
+#define HCLPP_FG(NDV)                                                  \
+    if (geom == kSpherical) { HCLPP_FR(NDV, kSpherical); }             \
+    else { HCLPP_FR(NDV, kCartesian); }
+    switch (nd) {
+    case 1: HCLPP_FG(1); break;
+    case 2: HCLPP_FG(2); break;
+    default: HCLPP_FG(3); break;
+    }
+#undef HCLPP_F4
+#undef HCLPP_FL
+#undef HCLPP_FR
+#undef HCLPP_FG
+}
+
+}  // namespace hclpp

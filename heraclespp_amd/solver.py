@@ -2,15 +2,15 @@
 :354-531 (loop body) on one GPU.
 
 Two paths, both HIP:
-  * fused (default): one hclpp_step per iteration -- the fused
-    reconstruct/predict/Riemann/update kernel, the ghost fill and the device
-    clock (dt, t_end clipping, max_iter, the internal-energy check) -- with no
-    host synchronisation inside the loop;
+  * fused (default): one hclpp_step (hclpp_step_fx with one passive scalar)
+    per iteration -- the fused reconstruct/predict/Riemann/update kernel, the
+    ghost fill and the device clock (dt, t_end clipping, max_iter, the
+    internal-energy check) -- with no host synchronisation inside the loop;
   * operators: the reference operator sequence (time_step,
     minimum_internal_energy, MUSCL-Hancock reconstruction, Godunov update,
-    boundaries, cons->prim), one kernel per reference kernel, host-synchronised
-    on dt exactly like the reference.  Needed for passive scalars and used to
-    bisect parity.
+    pressure fix, boundaries, cons->prim), one kernel per reference kernel,
+    host-synchronised on dt exactly like the reference.  Needed for more than
+    one passive scalar and for the pressure fix, and used to bisect parity.
 Time outputs are not modelled (the reference clips dt to output times only
 when an output frequency is configured).
 """
@@ -46,14 +46,15 @@ class HydroSolver:
             raise ValueError("pressure_fix must be 'On' or 'Off'")
         self.pressure_fix = pressure_fix == "On"  # main.cpp:459 (operator path: it needs the face states)
         self.eps_pf = float(eps_pf)
-        if precision == "fp32" and (not fused or nfx or self.pressure_fix):
-            raise ValueError("FP32 storage is a fused-step mode without passive scalars or pressure fix")
+        if precision == "fp32" and (not fused or nfx > 1 or self.pressure_fix):
+            raise ValueError("FP32 storage is a fused-step mode (at most one passive scalar, no pressure fix)")
         self.grid = grid
         self.eos = eos
         self.gravity = gravity
         self.cfl = float(cfl)
         self.nfx = nfx
-        self.fused = fused and nfx == 0 and not self.pressure_fix
+        # the fused step carries at most one passive scalar; the operator path takes any nfx
+        self.fused = fused and nfx <= 1 and not self.pressure_fix
         self.precision = precision
         self.ctx = Context(grid, eos, gravity, bc, bc_order, nfx, reconstruction, riemann_solver, precision)
         nd = grid.ndim
@@ -62,8 +63,9 @@ class HydroSolver:
         if precision == "fp32":
             f = lambda *lead: grid.zeros(*lead, dtype=torch.float32)  # noqa: E731
             self._cons32 = [(f(), f(nd), f()), (f(), f(nd), f())]
+            self._fxs32 = [f(nfx), f(nfx)] if nfx else None
         self._cur = 0
-        self.fx = z(max(nfx, 0)) if nfx else None
+        self._fxs = [z(nfx), z(nfx)] if nfx else None  # passive scalars (mass fractions), with the state
         self.u = z(nd)
         self.P = z()
         self.t = 0.0
@@ -76,7 +78,6 @@ class HydroSolver:
             self.u_rec = z(nd, nd, 2)
             self.P_rec = z(nd, 2)
             self.fx_rec = z(nfx, nd, 2) if nfx else None
-            self.fx_new = z(nfx) if nfx else None
 
     # ---- state access
     @property
@@ -90,6 +91,15 @@ class HydroSolver:
     @property
     def E(self):
         return self._cons[self._cur][2]
+
+    @property
+    def fx(self):
+        """Passive scalars (mass fractions) of the current state, or None when nfx = 0."""
+        return self._fxs[self._cur] if self.nfx else None
+
+    @property
+    def fx_new(self):
+        return self._fxs[1 - self._cur] if self.nfx else None
 
     def _r(self, ngeff) -> ctypes.Structure:
         return self.grid.range.with_ghosts(ngeff)._c()
@@ -137,7 +147,10 @@ class HydroSolver:
                 return n
 
     def _bufs(self):
-        return self._cons32 if self.precision == "fp32" else self._cons
+        fp32 = self.precision == "fp32"
+        cons = self._cons32 if fp32 else self._cons
+        fxs = (self._fxs32 if fp32 else self._fxs) if self.nfx else [None, None]
+        return [cons[i] + (fxs[i],) for i in range(2)]
 
     def _cast(self, src, dst):
         n = src.numel()
@@ -149,6 +162,8 @@ class HydroSolver:
         if self.precision == "fp32":
             for src, dst in zip(self._cons[self._cur], self._cons32[self._cur]):
                 self._cast(src, dst)
+            if self.nfx:
+                self._cast(self._fxs[self._cur], self._fxs32[self._cur])
             r, m, E = self._cons32[self._cur]
             check(lib().hclpp_clock_start_f32(c.handle, self.t, float(t_end), self.cfl, int(max_iter), _ptr(r), _ptr(m),
                                               _ptr(E)))
@@ -158,13 +173,21 @@ class HydroSolver:
                                           _ptr(E)))
 
     def _launch_step(self, a, b):
-        step = lib().hclpp_step_f32 if self.precision == "fp32" else lib().hclpp_step
-        check(step(self.ctx.handle, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(b[0]), _ptr(b[1]), _ptr(b[2])))
+        fp32 = self.precision == "fp32"
+        if self.nfx:
+            step = lib().hclpp_step_fx_f32 if fp32 else lib().hclpp_step_fx
+            check(step(self.ctx.handle, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]), _ptr(b[0]), _ptr(b[1]), _ptr(b[2]),
+                       _ptr(b[3])))
+        else:
+            step = lib().hclpp_step_f32 if fp32 else lib().hclpp_step
+            check(step(self.ctx.handle, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(b[0]), _ptr(b[1]), _ptr(b[2])))
 
     def _sync_back(self):
         if self.precision == "fp32":
             for src, dst in zip(self._cons32[self._cur], self._cons[self._cur]):
                 self._cast(src, dst)
+            if self.nfx:
+                self._cast(self._fxs32[self._cur], self._fxs[self._cur])
 
     def _run_fused(self, t_end, max_iter, check_every):
         it0 = self.iter
@@ -269,8 +292,6 @@ class HydroSolver:
         check(L.hclpp_fill_ghosts(h, c._chk(nxt[0]), c._chk(nxt[1], (nd,)), c._chk(nxt[2]),
                                   c._fx(self.fx_new, (nfx,), "fx_new")))
         self._cur = 1 - self._cur
-        if nfx:
-            self.fx, self.fx_new = self.fx_new, self.fx
         self.update_primitives()
         self._refresh_gravity()  # :528
         self.t += dt

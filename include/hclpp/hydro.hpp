@@ -560,8 +560,8 @@ public:
 };
 
 // ------------------------------------------------------------ main.cpp:305-311, 354-531 (fused path)
-// Owns a double-buffered conservative state and runs fused steps with the
-// device clock: no host synchronisation inside the loop.
+// Owns a double-buffered conservative state (and passive scalars, nfx <= 1) and
+// runs fused steps with the device clock: no host synchronisation inside the loop.
 template <class Gravity>
 class HydroDriver
 {
@@ -571,37 +571,44 @@ class HydroDriver
     std::array<KV_double_3d, 2> m_rho;
     std::array<KV_double_4d, 2> m_rhou;
     std::array<KV_double_3d, 2> m_E;
+    std::array<KV_double_4d, 2> m_fx;  // mass fractions (nfx components), allocated when nfx > 0
+    int m_nfx;
     int m_cur = 0;
     double m_t = 0;
     int m_iter = 0;
 
 public:
     HydroDriver(Grid const& grid, PerfectGas const& eos, Gravity const& gravity, std::string const& reconstruction, std::string const& riemann,
-                DistributedBoundaryCondition const& bcs, double cfl)
-        : m_grid(grid), m_cfl(cfl)
+                DistributedBoundaryCondition const& bcs, double cfl, int nfx = 0)
+        : m_grid(grid), m_cfl(cfl), m_nfx(nfx)
     {
+        if (nfx < 0 || nfx > 1) throw std::runtime_error("the fused driver carries at most one passive scalar (use the operators)");
         int const lim = hclpp_limiter_from_name(reconstruction.c_str());
         if (lim < 0) throw std::runtime_error("Unknown face reconstruction algorithm: " + reconstruction + ".");
         int const rs = hclpp_riemann_from_name(riemann.c_str());
         if (rs < 0) throw std::runtime_error("Invalid riemann solver: " + riemann + ".");
         auto const [kind, g, m] = detail::gravity_params(gravity);
-        m_ctx = grid.context(eos, kind, g, m, bcs.kinds(), bcs.order(), 0, lim, rs);
+        m_ctx = grid.context(eos, kind, g, m, bcs.kinds(), bcs.order(), nfx, lim, rs);
         auto const& n = grid.Nx_local_wg;
         for (int b = 0; b < 2; ++b) {
             m_rho[b] = KV_double_3d("rho", n[0], n[1], n[2]);
             m_rhou[b] = KV_double_4d("rhou", n[0], n[1], n[2], grid.ndim);
             m_E[b] = KV_double_3d("E", n[0], n[1], n[2]);
+            if (nfx > 0) m_fx[b] = KV_double_4d("fx", n[0], n[1], n[2], nfx);
         }
     }
 
     // main.cpp:305-311: prim -> cons on the interior, boundaries on the conservative state
-    void init_from_prim(KV_double_3d const& rho, KV_double_4d const& u, KV_double_3d const& P)
+    // fx: the passive scalars (mass fractions) when the driver has nfx > 0
+    void init_from_prim(KV_double_3d const& rho, KV_double_4d const& u, KV_double_3d const& P, KV_double_4d const& fx = {})
     {
+        if (m_nfx > 0 && fx.size() != m_fx[0].size()) throw std::runtime_error("fx: expected nfx mass fractions per cell");
         m_cur = 0;
         check_hip(hipMemcpy(m_rho[0].data(), rho.data(), rho.size() * sizeof(double), hipMemcpyDeviceToDevice), "hipMemcpy");
+        if (m_nfx > 0) check_hip(hipMemcpy(m_fx[0].data(), fx.data(), fx.size() * sizeof(double), hipMemcpyDeviceToDevice), "hipMemcpy");
         hclpp_range const r = m_grid.range.no_ghosts().cells();
         check(hclpp_conv_prim_to_cons(m_ctx, &r, m_rho[0].data(), u.data(), P.data(), m_rhou[0].data(), m_E[0].data()));
-        check(hclpp_fill_ghosts(m_ctx, m_rho[0].data(), m_rhou[0].data(), m_E[0].data(), nullptr));
+        check(hclpp_fill_ghosts(m_ctx, m_rho[0].data(), m_rhou[0].data(), m_E[0].data(), m_nfx > 0 ? m_fx[0].data() : nullptr));
         m_t = 0;
         m_iter = 0;
     }
@@ -616,7 +623,12 @@ public:
         for (;;) {
             for (int k = 0; k < check_every; ++k, ++launched) {
                 int const a = (cur0 + launched) % 2, b = 1 - a;
-                check(hclpp_step(m_ctx, m_rho[a].data(), m_rhou[a].data(), m_E[a].data(), m_rho[b].data(), m_rhou[b].data(), m_E[b].data()));
+                if (m_nfx > 0)
+                    check(hclpp_step_fx(m_ctx, m_rho[a].data(), m_rhou[a].data(), m_E[a].data(), m_fx[a].data(), m_rho[b].data(),
+                                        m_rhou[b].data(), m_E[b].data(), m_fx[b].data()));
+                else
+                    check(hclpp_step(m_ctx, m_rho[a].data(), m_rhou[a].data(), m_E[a].data(), m_rho[b].data(), m_rhou[b].data(),
+                                     m_E[b].data()));
             }
             check(hclpp_clock_read(m_ctx, &clk));
             if (clk.done || clk.status) break;
@@ -635,6 +647,7 @@ public:
     KV_double_3d const& rho() const { return m_rho[m_cur]; }
     KV_double_4d const& rhou() const { return m_rhou[m_cur]; }
     KV_double_3d const& E() const { return m_E[m_cur]; }
+    KV_double_4d const& fx() const { return m_fx[m_cur]; }
     double time() const { return m_t; }
     int iter() const { return m_iter; }
 };

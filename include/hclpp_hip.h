@@ -169,9 +169,18 @@ int hclpp_clock_start(hclpp_ctx* ctx, double t0, double t_end, double cfl, int m
 int hclpp_clock_read(hclpp_ctx* ctx, hclpp_clock* out);
 /* One fused step: reads (rho, rhou, E) ghost-filled, writes the new state,
  * ghosts included, to the *_new arrays (which must not alias the inputs).
- * FP64 storage; HCLPP_ERR_UNSUPPORTED when the context has nfx > 0. */
+ * FP64 storage; HCLPP_ERR_UNSUPPORTED when the context has nfx > 0 (see hclpp_step_fx). */
 int hclpp_step(hclpp_ctx* ctx, const double* rho, const double* rhou, const double* E, double* rho_new, double* rhou_new,
                double* E_new);
+/* hclpp_step carrying the context's passive scalar (nfx = 1; fx and fx_new hold nfx arrays of
+ * mass fractions, layout [f][cell], ghosts filled on input as hclpp_fill_ghosts leaves them):
+ * the main loop body of main.cpp:426-457 with nfx > 0, i.e. the scalar reconstructed with the
+ * state (hydro_reconstruction.hpp:99-109), carried through the Hancock half step as a partial
+ * density (extrapolation_reconstruction.hpp:87-88, 105-108, 315-337), upwinded with the mass
+ * flux and clamped to [0, 1] in the update (godunov_scheme.hpp:122, 263-301).  nfx > 1:
+ * HCLPP_ERR_UNSUPPORTED (the operator entry points take any nfx). */
+int hclpp_step_fx(hclpp_ctx* ctx, const double* rho, const double* rhou, const double* E, const double* fx, double* rho_new,
+                  double* rhou_new, double* E_new, double* fx_new);
 
 /* ---- FP32 storage (mixed precision) fused step ----
  * Same step as hclpp_step with the state stored and streamed in FP32 (half
@@ -183,6 +192,9 @@ int hclpp_step(hclpp_ctx* ctx, const double* rho, const double* rhou, const doub
 int hclpp_clock_start_f32(hclpp_ctx* ctx, double t0, double t_end, double cfl, int max_iter, const float* rho, const float* rhou,
                           const float* E);
 int hclpp_step_f32(hclpp_ctx* ctx, const float* rho, const float* rhou, const float* E, float* rho_new, float* rhou_new, float* E_new);
+/* hclpp_step_fx with FP32 storage (the scalar is stored and computed in FP32). */
+int hclpp_step_fx_f32(hclpp_ctx* ctx, const float* rho, const float* rhou, const float* E, const float* fx, float* rho_new,
+                      float* rhou_new, float* E_new, float* fx_new);
 /* element-wise conversions between FP64 and FP32 device arrays of n values */
 int hclpp_cast_f64_to_f32(hclpp_ctx* ctx, const double* in, float* out, long long n);
 int hclpp_cast_f32_to_f64(hclpp_ctx* ctx, const float* in, double* out, long long n);

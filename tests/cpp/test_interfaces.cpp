@@ -349,6 +349,109 @@ static void test_pressure_fix_loop()
     or_destroy(o);
 }
 
+// main.cpp:305-531 with one passive scalar (nfx = 1, a tracer of the dense band): the reference
+// operator sequence through the interface classes and the fused driver (hclpp_step_fx), both
+// against the oracle driver; every array, the scalar included, must be bit-identical.
+static void test_passive_scalar_loop()
+{
+    int const nd = 2, n = 32, nfx = 1, nsteps = 10;
+    double const cfl = 0.4, gamma = 1.4;
+    Grid const grid(nd, {n, n, 1}, 2, Geometry::Geom_cartesian, {0, 0, 0}, {1, 1, 1});
+    PerfectGas const eos(gamma, 1.);
+    UniformGravity const g({0.0, 0.0, 0.0});
+    auto const& w = grid.Nx_local_wg;
+    size_t const cells = static_cast<size_t>(w[0]) * w[1] * w[2];
+    std::vector<double> hrho(cells, 0.), hu(cells * nd, 0.), hP(cells, 0.), hfx(cells * nfx, 0.);
+    for (int j = 2; j < n + 2; ++j)
+        for (int i = 2; i < n + 2; ++i) {
+            size_t const c = i + w[0] * j;
+            double const x = (i - 1.5) / n, y = (j - 1.5) / n;
+            bool const band = std::abs(y - 0.5) < 0.25;
+            hrho[c] = band ? 2.0 : 1.0;
+            hu[c] = band ? 0.5 : -0.5;
+            hu[c + cells] = 0.01 * std::sin(4 * M_PI * x) * std::sin(5.0 * x + 2.0 * y);
+            hP[c] = 2.5;
+            hfx[c] = band ? 1.0 : 0.0;
+        }
+    or_config oc {};
+    oc.ndim = nd;
+    oc.geom = OR_GEOM_CARTESIAN;
+    oc.limiter = OR_LIM_VANLEER;
+    oc.riemann = OR_RS_HLLC;
+    oc.gravity = OR_GRAV_UNIFORM;
+    oc.nfx = nfx;
+    oc.ng = 2;
+    oc.n[0] = n;
+    oc.n[1] = n;
+    oc.n[2] = 1;
+    for (int f = 0; f < 6; ++f) {
+        oc.bc[f] = OR_BC_PERIODIC;
+        oc.bc_order[f] = f;
+    }
+    oc.gamma = gamma;
+    oc.mmw = 1.;
+    for (int d = 0; d < 3; ++d) oc.x[d] = grid.x[d].data();
+    or_ctx* o = or_create(&oc);
+    or_init_from_prim(o, hrho.data(), hu.data(), hP.data(), hfx.data());
+    for (int s = 0; s < nsteps; ++s) {
+        double dt;
+        int ex;
+        or_step(o, cfl, 1e30, nsteps, &dt, &ex);
+    }
+    size_t const bytes = cells * 8;
+    // the reference sequence through the interfaces
+    KV_double_3d rho("rho", w[0], w[1], w[2]), P("P", w[0], w[1], w[2]), E("E", w[0], w[1], w[2]);
+    KV_double_4d u("u", w[0], w[1], w[2], nd), rhou("rhou", w[0], w[1], w[2], nd), fx("fx", w[0], w[1], w[2], nfx);
+    KV_double_3d rho_new("rho_new", w[0], w[1], w[2]), E_new("E_new", w[0], w[1], w[2]);
+    KV_double_4d rhou_new("rhou_new", w[0], w[1], w[2], nd), fx_new("fx_new", w[0], w[1], w[2], nfx);
+    KV_double_5d rho_rec("rho_rec", w[0], w[1], w[2], 2, nd), E_rec("E_rec", w[0], w[1], w[2], 2, nd), P_rec("P_rec", w[0], w[1], w[2], 2, nd);
+    KV_double_6d rhou_rec("rhou_rec", w[0], w[1], w[2], 2, nd, nd), u_rec("u_rec", w[0], w[1], w[2], 2, nd, nd),
+            fx_rec("fx_rec", w[0], w[1], w[2], 2, nd, nfx);
+    rho.copy_from_host(hrho);
+    u.copy_from_host(hu);
+    P.copy_from_host(hP);
+    fx.copy_from_host(hfx);
+    DistributedBoundaryCondition const bcs(grid, {"Periodic", "Periodic", "Periodic", "Periodic"});
+    auto reconstruction = std::make_unique<MUSCLHancockHydroReconstruction<UniformGravity>>(
+            factory_face_reconstruction("VanLeer"), std::make_unique<HancockExtrapolationReconstruction<UniformGravity>>(eos), eos, P_rec, u_rec);
+    auto godunov = factory_godunov_scheme<UniformGravity>("HLLC", eos);
+    conv_prim_to_cons(grid.range.no_ghosts(), eos, grid, rho, u, P, rhou, E);
+    bcs(grid, rho, rhou, E, fx);
+    conv_cons_to_prim(grid.range.all_ghosts(), eos, grid, rho, rhou, E, u, P);
+    for (int s = 0; s < nsteps; ++s) {
+        double const dt = cfl * time_step(grid.range.all_ghosts(), eos, grid, rho, u, P);
+        reconstruction->execute(grid.range.with_ghosts(1), grid, g, dt / 2, rho, u, P, fx, rho_rec, rhou_rec, E_rec, fx_rec);
+        godunov->execute(grid.range.no_ghosts(), grid, g, dt, rho, rhou, E, fx, rho_rec, rhou_rec, E_rec, fx_rec, rho_new, rhou_new, E_new, fx_new);
+        bcs(grid, rho_new, rhou_new, E_new, fx_new);
+        std::swap(rho, rho_new);
+        std::swap(rhou, rhou_new);
+        std::swap(E, E_new);
+        std::swap(fx, fx_new);
+        conv_cons_to_prim(grid.range.all_ghosts(), eos, grid, rho, rhou, E, u, P);
+    }
+    EXPECT(std::memcmp(rho.to_host().data(), or_state(o, 0), bytes) == 0, "scalar loop: rho differs from the oracle");
+    EXPECT(std::memcmp(rhou.to_host().data(), or_state(o, 1), bytes * nd) == 0, "scalar loop: rhou differs from the oracle");
+    EXPECT(std::memcmp(E.to_host().data(), or_state(o, 2), bytes) == 0, "scalar loop: E differs from the oracle");
+    EXPECT(std::memcmp(fx.to_host().data(), or_state(o, 3), bytes * nfx) == 0, "scalar loop: fx differs from the oracle");
+    // the fused driver
+    HydroDriver<UniformGravity> drv(grid, eos, g, "VanLeer", "HLLC", bcs, cfl, nfx);
+    KV_double_3d rho0("rho0", w[0], w[1], w[2]), P0("P0", w[0], w[1], w[2]);
+    KV_double_4d u0("u0", w[0], w[1], w[2], nd), fx0("fx0", w[0], w[1], w[2], nfx);
+    rho0.copy_from_host(hrho);
+    u0.copy_from_host(hu);
+    P0.copy_from_host(hP);
+    fx0.copy_from_host(hfx);
+    drv.init_from_prim(rho0, u0, P0, fx0);
+    int const steps = drv.run(1e30, nsteps);
+    EXPECT(steps == nsteps, "steps %d", steps);
+    EXPECT(std::memcmp(drv.rho().to_host().data(), or_state(o, 0), bytes) == 0, "fused scalar run: rho differs from the oracle");
+    EXPECT(std::memcmp(drv.rhou().to_host().data(), or_state(o, 1), bytes * nd) == 0, "fused scalar run: rhou differs from the oracle");
+    EXPECT(std::memcmp(drv.E().to_host().data(), or_state(o, 2), bytes) == 0, "fused scalar run: E differs from the oracle");
+    EXPECT(std::memcmp(drv.fx().to_host().data(), or_state(o, 3), bytes * nfx) == 0, "fused scalar run: fx differs from the oracle");
+    EXPECT(drv.time() == or_time(o), "time %.17g vs %.17g", drv.time(), or_time(o));
+    or_destroy(o);
+}
+
 int main()
 {
     try {
@@ -357,6 +460,7 @@ int main()
         test_operator_loop();
         test_internal_mass_loop();
         test_pressure_fix_loop();
+        test_passive_scalar_loop();
     } catch (std::exception const& e) {
         std::printf("FAIL: exception %s\n", e.what());
         return 2;

@@ -38,3 +38,22 @@ def test_fp32_storage_accuracy(cuda, name):
     if nsteps is not None:
         assert out["fp32"][0] == out["fp64"][0] == nsteps
     assert l1 <= bound, l1
+
+
+def test_fp32_storage_passive_scalar(cuda):
+    """3D Rayleigh-Taylor with the heavy-fluid tracer (the reference input carries one scalar):
+    FP32 storage against the FP64 fused step, density and scalar L1 after 40 steps."""
+    s = setups.rayleigh_taylor(32, seed=1)
+    y = (s.rho > 1.5).astype(np.float64)[None]
+    out = {}
+    for prec in ("fp64", "fp32"):
+        sol = make_solver(s, precision=prec, nfx=1)
+        assert sol.fused
+        sol.init_from_prim(s.rho, s.u, s.P, fx=y)
+        assert sol.run(s.t_end, 40) == 40
+        out[prec] = (sol.rho.cpu().numpy()[s.interior()], sol.fx.cpu().numpy()[(0,) + s.interior()])
+    l1 = rel_l1(out["fp32"][0], out["fp64"][0])
+    l1x = rel_l1(out["fp32"][1], out["fp64"][1])
+    print(f"\n[mixed-fx] rayleigh_taylor3d 32^3 40 steps: relL1(rho)={l1:.3e} relL1(fx)={l1x:.3e}")
+    assert l1 <= 1e-5, l1
+    assert l1x <= 1e-4, l1x

@@ -212,19 +212,25 @@ def test_boundaries(cuda, nd, bcs, order):
         assert np.array_equal(_n(dev[k]), ref[k]), k
 
 
-def test_fused_step_rejects_passive_scalars(cuda):
-    """The fused step carries no passive scalars: the C-ABI refuses before touching any array."""
+def test_fused_step_passive_scalar_rules(cuda):
+    """The fused step carries one passive scalar (hclpp_step_fx); the C-ABI refuses, before
+    touching any array, hclpp_step on a context with nfx > 0 and hclpp_step_fx with nfx > 1
+    (the solver routes nfx > 1 to the operator path)."""
     import ctypes
 
     from heraclespp_amd import _lib, setups
     from tests.helpers import make_solver
 
     s = setups.kelvin_helmholtz(16)
-    sol = make_solver(s, fused=True, nfx=1)
-    assert not sol.fused  # the driver routes nfx > 0 to the operator path
-    fake = [ctypes.c_void_p(64 * (k + 1)) for k in range(6)]
-    with pytest.raises(_lib.HclppError, match="passive scalars"):
-        _lib.check(_lib.lib().hclpp_step(sol.ctx.handle, *fake))
+    one = make_solver(s, fused=True, nfx=1)
+    assert one.fused
+    fake = [ctypes.c_void_p(64 * (k + 1)) for k in range(8)]
+    with pytest.raises(_lib.HclppError, match="hclpp_step_fx"):
+        _lib.check(_lib.lib().hclpp_step(one.ctx.handle, *fake[:6]))
+    two = make_solver(s, fused=True, nfx=2)
+    assert not two.fused
+    with pytest.raises(_lib.HclppError, match="at most one passive scalar"):
+        _lib.check(_lib.lib().hclpp_step_fx(two.ctx.handle, *fake))
 
 
 @pytest.mark.parametrize("fused", [True, False])

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from heraclespp_amd import setups
+from oracle import oracle as orc
 from tests.helpers import make_oracle, make_solver, rel_l1, run_oracle
 
 pytestmark = pytest.mark.gpu
@@ -52,3 +53,84 @@ def test_step_parity(cuda, name, fused):
     for k, (mx, l1) in res.items():
         assert l1 <= 1e-12, (k, l1)
         assert mx == 0.0, (k, mx)
+
+
+# ---------------------------------------------------------------- one passive scalar
+# (fused: hclpp_step_fx; operators: the reference kernel sequence with nfx = 1)
+
+def _tracer_above(threshold):
+    return lambda s, rng: (s.rho > threshold).astype(np.float64)
+
+
+def _random_scalar(s, rng):
+    return 0.1 + 0.8 * rng.random(s.rho.shape)
+
+
+def _inner_radial_half(s, rng):
+    y = np.zeros(s.rho.shape)
+    y[..., : s.ng + s.n[0] // 2] = 1.0
+    return y
+
+
+FX_CASES = {
+    "sod1d_tracer": (lambda: setups.sod(1000), None, {}, _tracer_above(0.5)),
+    "kh2d_band_tracer": (lambda: setups.kelvin_helmholtz(64, seed=1), 20, {}, _tracer_above(1.5)),
+    "kh2d_random_scalar_hll_minmod": (lambda: setups.kelvin_helmholtz(40, seed=6), 10, {"riemann": "HLL", "limiter": "Minmod"},
+                                      _random_scalar),
+    "rt3d_heavy_fluid": (lambda: setups.rayleigh_taylor(16, seed=2), 8, {}, _tracer_above(1.5)),
+    "sedov3d_inner_half": (lambda: setups.sedov(24, 12, 16, t_end=1.0), 8, {}, _inner_radial_half),
+    "smooth2d_random_scalar_vanalbada": (lambda: setups.smooth_euler(48, seed=3), 12, {}, _random_scalar),
+}
+
+
+@pytest.mark.parametrize("name", list(FX_CASES))
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "operators"])
+def test_step_parity_passive_scalar(cuda, name, fused):
+    make, nsteps, kw, scalar = FX_CASES[name]
+    s = make()
+    y = scalar(s, np.random.default_rng(7))[None]
+    n = nsteps if nsteps is not None else 1 << 30
+    print(f"\n[parity-fx] case={name} fused={fused} n={s.n} steps={nsteps}", flush=True)
+    o = make_oracle(s, nfx=1, **kw)
+    o.init_from_prim(s.rho, s.u, s.P, y)
+    for _ in range(n):
+        _, ex = o.step(s.cfl, s.t_end, n)
+        if ex:
+            break
+    sol = make_solver(s, fused=fused, nfx=1, **kw)
+    assert sol.fused == fused
+    sol.init_from_prim(s.rho, s.u, s.P, fx=y)
+    steps = sol.run(s.t_end, n)
+    assert steps == o.iter
+    assert sol.t == o.t
+    for nm, gpu in (("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E), ("fx", sol.fx)):
+        got = gpu.cpu().numpy()
+        ref = o.state(nm)
+        print(f"[parity-fx] {name} {nm}: maxabs={float(np.max(np.abs(got - ref))):.3e}", flush=True)
+        np.testing.assert_array_equal(got, ref, err_msg=nm)
+
+
+@pytest.mark.parametrize("nd,n,steps", [(1, [64], 20), (2, [24, 16], 10)])
+def test_step_parity_passive_scalar_spherical(cuda, nd, n, steps):
+    """Spherical 1D / 2D (the march dim carries the geometric pressure source) with a scalar."""
+    import heraclespp_amd as H
+    from heraclespp_amd.solver import HydroSolver
+
+    lo, hi = (0.3, 0.2, 0.0), (1.3, np.pi - 0.2, 2 * np.pi)
+    bc = ["Reflexive", "NullGradient", "Reflexive", "Reflexive"][:2 * nd]
+    o = orc.Oracle(nd, n, geom="Spherical", xmin=lo, xmax=hi, bc=bc, limiter="VanLeer", riemann="HLLC", nfx=1)
+    rng = np.random.default_rng(nd)
+    rho = 1.0 + 0.4 * rng.random(o.shape3)
+    u = 0.2 * rng.standard_normal((nd,) + o.shape3)
+    P = 1.0 + 0.3 * rng.random(o.shape3)
+    y = rng.random((1,) + o.shape3)
+    o.init_from_prim(rho, u, P, y)
+    for _ in range(steps):
+        o.step(0.4, 1e30, steps)
+    g = H.Grid(nd, n, 2, "Spherical", lo, hi)
+    sol = HydroSolver(g, H.PerfectGas(1.4, 1.0), reconstruction="VanLeer", riemann_solver="HLLC", bc=bc, cfl=0.4, nfx=1)
+    assert sol.fused
+    sol.init_from_prim(rho, u, P, fx=y)
+    assert sol.run(1e30, steps) == steps
+    for nm, gpu in (("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E), ("fx", sol.fx)):
+        np.testing.assert_array_equal(gpu.cpu().numpy(), o.state(nm), err_msg=nm)
