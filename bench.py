@@ -256,6 +256,8 @@ def main():
     if clk.iter != a.warmup + a.steps:
         raise RuntimeError(f"expected {a.warmup + a.steps} steps, clock counted {clk.iter}")
 
+    seglen, nwg, occ = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()
+    _lib.check(L.hclpp_fused_plan(solver.ctx.handle, ctypes.byref(seglen), ctypes.byref(nwg), ctypes.byref(occ)))
     wall = t1 - t0
     value = cells * a.steps / wall / 1e9
     kernel_s = kms.value / 1e3 / max(nl.value, 1)
@@ -292,6 +294,7 @@ def main():
             "riemann_solver": s.riemann_solver,
             "storage": a.precision,
             "path": "fused (hclpp_step: march kernel + ghost fill + device dt)",
+            "launch": {"segment_planes": seglen.value, "workgroups": nwg.value, "resident_per_cu": occ.value},
         },
         "roofline": {
             "bound": "hbm",

@@ -31,10 +31,11 @@ EXPORTS = [
     "hclpp_limiter_from_name", "hclpp_riemann_from_name", "hclpp_bc_from_name",
     "hclpp_conv_prim_to_cons", "hclpp_conv_cons_to_prim", "hclpp_face_reconstruction", "hclpp_hancock_extrapolation",
     "hclpp_hydro_reconstruction", "hclpp_godunov", "hclpp_time_step", "hclpp_min_internal_energy", "hclpp_integrate",
-    "hclpp_fill_ghosts", "hclpp_clock_start", "hclpp_clock_read", "hclpp_step", "hclpp_step_fx", "hclpp_clock_start_f32", "hclpp_step_f32", "hclpp_step_fx_f32",
+    "hclpp_fill_ghosts", "hclpp_clock_start", "hclpp_clock_read", "hclpp_step", "hclpp_step_fx",
+    "hclpp_clock_start_f32", "hclpp_step_f32", "hclpp_step_fx_f32",
     "hclpp_cast_f64_to_f32", "hclpp_cast_f32_to_f64", "hclpp_step_dt", "hclpp_clock_record_dt", "hclpp_step_adjoint",
     "hclpp_steps_dt", "hclpp_steps_adjoint", "hclpp_pressure_fix", "hclpp_internal_mass_gravity", "hclpp_gravity_table",
-    "hclpp_profile_begin", "hclpp_profile_end",
+    "hclpp_profile_begin", "hclpp_profile_end", "hclpp_fused_plan",
 ]
 
 
@@ -123,6 +124,7 @@ def lib() -> ctypes.CDLL:
         "hclpp_clock_start_f32": [vp, d, d, d, ip, dp, dp, dp],
         "hclpp_step_f32": [vp, dp, dp, dp, dp, dp, dp],
         "hclpp_step_fx_f32": [vp, dp, dp, dp, dp, dp, dp, dp, dp],
+        "hclpp_fused_plan": [vp, ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ip)],
         "hclpp_cast_f64_to_f32": [vp, dp, dp, ctypes.c_longlong],
         "hclpp_cast_f32_to_f64": [vp, dp, dp, ctypes.c_longlong],
         "hclpp_step_dt": [vp, ctypes.c_double, dp, dp, dp, dp, dp, dp],

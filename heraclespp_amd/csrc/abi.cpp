@@ -48,6 +48,8 @@ struct hclpp_ctx {
     double* fx_old = nullptr;       // Hancock scratch for passive scalars
     int64_t fx_old_cap = 0;
     GhostShell shell{};
+    MarchPlan march {};  // launch plan of the fused kernel
+    int march_occ = 0;    // its resident workgroups per CU
     int64_t fused_nb = 0;
     double* gtab = nullptr;         // radial gravity table (inside tables; rebuilt for internal-mass gravity)
     double* shells = nullptr;       // internal-mass gravity: per-layer sums of rho dV and dV
@@ -295,7 +297,16 @@ int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out)
         e = hipMalloc(&c->ghost_partials, sizeof(double) * c->ghost_cap);
         if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(ghost partials)"));
     }
-    c->fused_nb = fused_blocks(G, nd, nullptr);
+    {  // the fused kernel this context launches, and how many of its workgroups the chip holds
+        int dev = 0, ncu = 0;
+        e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return cleanup(hip_fail(e, "hipDeviceGetAttribute(multiProcessorCount)"));
+        int const occ = march_occupancy(nd, cfg->geom, cfg->riemann, cfg->limiter, cfg->precision == HCLPP_FP32_STORAGE, cfg->nfx == 1);
+        c->march = plan_march(G, nd, occ, ncu);
+        c->march_occ = occ;
+    }
+    c->fused_nb = c->march.nb;
     Range3 all;
     for (int d = 0; d < 3; ++d) {
         all.b[d] = 0;
@@ -570,8 +581,8 @@ static int fused_step(hclpp_ctx* c, const S* rho, const S* rhou, const S* E, con
     }
     bool const timed = c->prof_count < c->prof_max;
     if (timed) HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count], c->stream));
-    if (nfx > 0) launch_fused_step_fx(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, eos, A, c->stream);
-    else launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, eos, A, c->stream);
+    if (nfx > 0) launch_fused_step_fx(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->march, eos, A, c->stream);
+    else launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->march, eos, A, c->stream);
     HCLPP_LAUNCHED("fused step");
     if (timed) {
         HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count + 1], c->stream));
@@ -796,6 +807,15 @@ int hclpp_profile_end(hclpp_ctx* c, double* kernel_ms, int* launches)
     *launches = c->prof_count;
     c->prof_max = 0;
     c->prof_count = 0;
+    return HCLPP_OK;
+}
+
+int hclpp_fused_plan(hclpp_ctx* c, int* seglen, long long* workgroups, int* blocks_per_cu)
+{
+    if (!c || !seglen || !workgroups || !blocks_per_cu) return fail(HCLPP_ERR_ARG, "null argument");
+    *seglen = c->march.L.seglen;
+    *workgroups = c->march.nb;
+    *blocks_per_cu = c->march_occ;
     return HCLPP_OK;
 }
 

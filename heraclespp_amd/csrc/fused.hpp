@@ -65,19 +65,36 @@ struct MarchShape<1> {  // 64 independent segments of 16 cells per workgroup
 };
 template <>
 struct MarchShape<2> {  // a 64-column row (62 outputs + halo) marching along y
-    static constexpr int HX = 64, HY = 1, NT = 64, SEG = 32;
+    static constexpr int HX = 64, HY = 1, NT = 64;
 };
 template <>
-struct MarchShape<3> {  // 32 x 8 columns (30 x 6 outputs + halo) marching along z
-    static constexpr int HX = 32, HY = 8, NT = 256, SEG = 64;
+struct MarchShape<3> {  // 16 x 16 columns (14 x 14 outputs + halo) marching along z
+    // The halo columns repeat the predictor of their neighbours' faces: 16 x 16 computes 256
+    // predictors for 196 outputs where 32 x 8 computed 256 for 180 (measured +11 % on RT 512^3).
+    static constexpr int HX = 16, HY = 16, NT = 256;
 };
 
-int64_t fused_blocks(const GridDev& G, int nd, FusedLayout* layout);
-void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, hipStream_t s);
-void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGasMixed& eos, const StepArraysF& A, hipStream_t s);
+// Launch plan of the fused march kernel for one context.
+struct MarchPlan {
+    FusedLayout L;
+    int64_t nb;  // workgroups (one partials row each)
+};
+// Tile counts and segment length for a kernel of which blocks_per_cu workgroups are resident
+// per CU, on ncu CUs (see step_fused.hip for the cost model).
+MarchPlan plan_march(const GridDev& G, int nd, int blocks_per_cu, int ncu);
+// Resident workgroups per CU of the k_step_march instance a configuration selects
+// (FP32 storage or not, one passive scalar or none).
+int march_occupancy(int nd, int geom, int rs, int lim, bool fp32, bool fx);
+void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGas& eos, const StepArrays& A,
+                       hipStream_t s);
+void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGasMixed& eos,
+                       const StepArraysF& A, hipStream_t s);
 // the fused step with one passive scalar (step_fused_fx.hip); A.fx and A.fx_new are set
-void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, hipStream_t s);
-void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGasMixed& eos, const StepArraysF& A, hipStream_t s);
+void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGas& eos, const StepArrays& A,
+                          hipStream_t s);
+void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGasMixed& eos,
+                          const StepArraysF& A, hipStream_t s);
+int march_occupancy_fx(int nd, int geom, int rs, int lim, bool fp32);
 GhostShell make_ghost_shell(const GridDev& G, int nd);
 int64_t ghost_blocks(const GhostShell& B);
 void launch_ghosts_rate(int nd, int geom, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const GhostShell& B, const StepArrays& A,

@@ -336,6 +336,16 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march
     __shared__ T s_st[NXCH][NL][HY][HX];
     __shared__ T s_fl[NXCH][NL][HY][HX];
     __shared__ double s_red[NT / 64];
+    // Per-thread parking of the values plane h-1 hands to plane h (its right march face state,
+    // its partial update and its left march flux, and the scalar's): written at the end of a
+    // plane, read back when plane h completes plane h-1.  Only the owning thread touches its
+    // slots, so no barrier is involved, and the registers are free across the plane's
+    // arithmetic.  Used where it pays (measured): the 3D FP32-storage kernel, whose register
+    // budget is set for 3 waves per SIMD (spills go from 40-152 to 0-92 B per lane; RT 512^3
+    // +6 %, spherical Sedov +26 %).  In FP64 and in 2D it gained nothing or lost.
+    constexpr bool kPark = !std::is_same_v<Eos, PerfectGas> && ND == 3;
+    constexpr int NPARK = kPark ? 3 * NV + 3 * NFX : 1;
+    __shared__ T s_park[NPARK][kPark ? NT : 1];
 
     DevClock const* clk = A.clock;
     if (clk->pub.done || clk->pub.status) return;
@@ -438,7 +448,7 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march
     // while plane h is processed; ring cell: plane h+1), so their latency hides behind a plane
     // of arithmetic instead of stalling the workgroup at the LDS barrier.
     State<ND, T> pm, p0, pp, c0, cp, cnext, rnext;
-    State<ND, T> fsR_prev, Fm_prev, part_prev;
+    State<ND, T> fsR_prev, Fm_prev, part_prev;  // in registers, or in s_park when kPark
     T old_rho_prev = 0, old_mom_prev = 0, src_prev = 0;
     // passive scalars: mass fractions of planes h-1, h, h+1 and the prefetch; the ring cell's
     // prefetch; the right march face of plane h-1, its partial-density update and the
@@ -470,6 +480,20 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march
             if constexpr (NFX > 0) rxnext = load_frac<NFX>(A, G, cell_at<ND>(G, ring_a, ring_b, zfirst));
         }
     }
+
+    if constexpr (kPark) {  // plane zfirst-1's primitives of the own column, where the loop reads them
+        if (in_block) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) s_prim[(zfirst - 1) & 1][v][(ND == 3) ? hy + 1 : 0][hx + 1] = pm[v];
+        }
+    }
+    auto park = [&](int slot, T v) {
+        if constexpr (kPark) s_park[slot][tid] = v;
+    };
+    auto unpark = [&](int slot) -> T {
+        if constexpr (kPark) return s_park[slot][tid];
+        else return T(0);
+    };
 
     for (int h = zfirst; h <= zlast; ++h) {
         int const pb = h & 1;
@@ -548,7 +572,12 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march
                     }
                 }
             }
-            nb3[MD][0] = pm;
+            if constexpr (kPark) {  // plane h-1 of the own column, still in the other buffer
+#pragma unroll
+                for (int v = 0; v < NV; ++v) nb3[MD][0][v] = s_prim[pb ^ 1][v][(ND == 3) ? hy + 1 : 0][hx + 1];
+            } else {
+                nb3[MD][0] = pm;
+            }
             nb3[MD][1] = pp;
             xnb[MD][0] = xm;
             xnb[MD][1] = xp;
@@ -661,10 +690,17 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march
 
         // ---- march-dim face h-1/2 and completion of plane h-1
         if (h >= zs && out_col) {
+            if constexpr (kPark) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) fsR_prev[v] = unpark(v);
+            }
             State<ND, T> const Fm = riemann<RS>(fsR_prev, fs[0][MD], MD, eos);
             X XFm;  // upwinded scalar flux of the face h-1/2
 #pragma unroll
-            for (int f = 0; f < NFX; ++f) XFm.f[f] = (Fm[0] < 0 ? xf[0][MD].f[f] : xfR_prev.f[f]) * Fm[0];
+            for (int f = 0; f < NFX; ++f) {
+                T const xr = kPark ? unpark(3 * NV + f) : xfR_prev.f[f];
+                XFm.f[f] = (Fm[0] < 0 ? xf[0][MD].f[f] : xr) * Fm[0];
+            }
             int const hm = h - 1;
             if (hm >= zs) {
                 int pi, pj, pk;
@@ -673,6 +709,18 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march
                 T const dtodv_m = T(dt_d / vol);
                 T const dS = T(face_area<ND, GEOM>(G, MD, pi, pj, pk));
                 T const dSp = T(face_area<ND, GEOM>(G, MD, ci, cj, ck));
+                if constexpr (kPark) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        part_prev[v] = unpark(NV + v);
+                        Fm_prev[v] = unpark(2 * NV + v);
+                    }
+#pragma unroll
+                    for (int f = 0; f < NFX; ++f) {
+                        xpart_prev.f[f] = unpark(3 * NV + NFX + f);
+                        ffm_prev.f[f] = unpark(3 * NV + 2 * NFX + f);
+                    }
+                }
                 State<ND, T> un = part_prev;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) un[v] += dtodv_m * ((Fm_prev[v] * dS) - (Fm[v] * dSp));
@@ -695,15 +743,35 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march
 #pragma unroll
                 for (int v = 0; v < NV; ++v) r_sum[v] += (double)un[v] * vol;
             }
-            Fm_prev = Fm;
-            ffm_prev = XFm;
+            if constexpr (kPark) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) park(2 * NV + v, Fm[v]);
+#pragma unroll
+                for (int f = 0; f < NFX; ++f) park(3 * NV + 2 * NFX + f, XFm.f[f]);
+            } else {
+                Fm_prev = Fm;
+                ffm_prev = XFm;
+            }
         }
 
         // ---- rotate the pipeline
-        fsR_prev = fs[1][MD];
-        part_prev = part;
-        xfR_prev = xf[1][MD];
-        xpart_prev = xpart;
+        if constexpr (kPark) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                park(v, fs[1][MD][v]);
+                park(NV + v, part[v]);
+            }
+#pragma unroll
+            for (int f = 0; f < NFX; ++f) {
+                park(3 * NV + f, xf[1][MD].f[f]);
+                park(3 * NV + NFX + f, xpart.f[f]);
+            }
+        } else {
+            fsR_prev = fs[1][MD];
+            part_prev = part;
+            xfR_prev = xf[1][MD];
+            xpart_prev = xpart;
+        }
         xm = x0;
         x0 = xp;
         old_rho_prev = old_rho;
@@ -728,23 +796,12 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march
     }
 }
 
-// Launch of k_step_march over the whole interior (one plane segment per workgroup column tile).
-template <class Eos, int ND, int GEOM, int RS, int LIM, int NFX>
-static void launch_march_t(const GridDev& G, const Eos& eos, const StepArraysT<typename Eos::real>& A, hipStream_t s)
+// Runtime dispatch over dimension, geometry, Riemann solver and limiter: calls
+// v.template run<ND, GEOM, RS, LIM>() for the configuration given.
+template <class V>
+static void march_visit(int nd, int geom, int rs, int lim, V&& v)
 {
-    using S = MarchShape<ND>;
-    FusedLayout L;
-    int64_t const nb = fused_blocks(G, ND, &L);
-    if (nb == 0) return;
-    hipLaunchKernelGGL((k_step_march<Eos, ND, GEOM, RS, LIM, NFX, S::HX, S::HY, S::NT>), dim3((unsigned)nb), dim3(S::NT), 0, s, G, eos, A, L);
-}
-
-// Runtime dispatch over dimension, geometry, Riemann solver and limiter.
-template <class Eos, int NFX>
-static void launch_march_any(int nd, int geom, int rs, int lim, const GridDev& G, const Eos& eos, const StepArraysT<typename Eos::real>& A,
-                             hipStream_t s)
-{
-#define HCLPP_F4(NDV, GV, RSV, LV) launch_march_t<Eos, NDV, GV, RSV, LV, NFX>(G, eos, A, s)
+#define HCLPP_F4(NDV, GV, RSV, LV) v.template run<NDV, GV, RSV, LV>()
 #define HCLPP_FL(NDV, GV, RSV)                                     \
     switch (lim) {                                                 \
     case kConstant: HCLPP_F4(NDV, GV, RSV, kConstant); break;      \
@@ -771,5 +828,39 @@ static void launch_march_any(int nd, int geom, int rs, int lim, const GridDev& G
 #undef HCLPP_FR
 #undef HCLPP_FG
 }
+
+// Launch of k_step_march with the context's plan (tile counts, segment length, workgroups).
+template <class Eos, int NFX>
+struct MarchLaunch {
+    const GridDev& G;
+    const MarchPlan& P;
+    const Eos& eos;
+    const StepArraysT<typename Eos::real>& A;
+    hipStream_t s;
+    template <int ND, int GEOM, int RS, int LIM>
+    void run() const
+    {
+        using S = MarchShape<ND>;
+        if (P.nb == 0) return;
+        hipLaunchKernelGGL((k_step_march<Eos, ND, GEOM, RS, LIM, NFX, S::HX, S::HY, S::NT>), dim3((unsigned)P.nb), dim3(S::NT), 0, s, G, eos,
+                           A, P.L);
+    }
+};
+
+// Resident workgroups per CU of one k_step_march instance (its registers and LDS decide).
+template <class Eos, int NFX>
+struct MarchOccupancy {
+    int* out;
+    template <int ND, int GEOM, int RS, int LIM>
+    void run() const
+    {
+        using S = MarchShape<ND>;
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step_march<Eos, ND, GEOM, RS, LIM, NFX, S::HX, S::HY, S::NT>, S::NT, 0) !=
+            hipSuccess)
+            n = 0;
+        *out = n;
+    }
+};
 
 }  // namespace hclpp

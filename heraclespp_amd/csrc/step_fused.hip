@@ -2,6 +2,8 @@
 // scalars), the ghost fill of the new state with its CFL rates, and the device clock.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "march.hpp"
 
 namespace hclpp {
@@ -177,54 +179,69 @@ __global__ __launch_bounds__(1024) void k_clock_advance(DevClock* c, const doubl
 }
 
 // ------------------------------------------------------------ launchers
-int64_t fused_blocks(const GridDev& G, int nd, FusedLayout* layout)
+// Launch plan.  Every workgroup of the march kernel carries the same work: one tile through a
+// segment of `seg` planes, plus the two halo planes zs-1 and ze its predictor also visits.  The
+// chip holds `resident` = blocks_per_cu x ncu workgroups at once, so a launch of nb workgroups
+// runs in ceil(nb / resident) rounds of (seg + 2) planes each.  The plan takes the segment
+// length (kMinSeg planes at least) with the fewest rounds x (seg + 2): e.g. 2D 2048^2 in FP64
+// (34 tiles, two resident single-wave workgroups per SIMD) gets 35-plane segments in one round
+// of 2006 workgroups, where 32-plane segments need 2176 workgroups and a second round for 6 %
+// of the work.  The split changes which workgroup computes a cell, not how: results do not
+// depend on it (the conserved totals, summed per workgroup, only change their summation order).
+MarchPlan plan_march(const GridDev& G, int nd, int blocks_per_cu, int ncu)
 {
-    FusedLayout L{};
-    int64_t nb = 0;
-    auto shape = [&](int HX, int HY, int NT, int SEG) {
-        L.seglen = SEG;
-        int const nm = G.n[nd - 1];
-        int const nseg = (nm + SEG - 1) / SEG;
-        if (nd == 1) {
-            L.ntile[0] = 1;
-            L.ntile[1] = 1;
-            nb = (nseg + NT - 1) / NT;
-        } else {
-            int const TX = HX - 2;
-            int const TY = (nd == 3) ? HY - 2 : 1;
-            L.ntile[0] = (G.n[0] + TX - 1) / TX;
-            L.ntile[1] = (nd == 3) ? (G.n[1] + TY - 1) / TY : 1;
-            nb = (int64_t)L.ntile[0] * L.ntile[1] * nseg;
-        }
-    };
-    // Shorter segments for small grids: halve the march length until the launch holds at least
-    // kFillWaves wavefronts (two per SIMD of the 1024 SIMDs), down to kMinSeg planes.  Every
-    // cell is computed by the same expressions whatever the split, so results do not change.
-    constexpr int64_t kFillWaves = 2048;
+    MarchPlan P{};
+    int const nm = G.n[nd - 1];
+    if (nd == 1) {
+        using S = MarchShape<1>;
+        P.L.seglen = S::SEG;
+        P.L.ntile[0] = 1;
+        P.L.ntile[1] = 1;
+        int const nseg = (nm + S::SEG - 1) / S::SEG;
+        P.nb = (nseg + S::NT - 1) / S::NT;
+        return P;
+    }
+    int const HX = nd == 2 ? MarchShape<2>::HX : MarchShape<3>::HX;
+    int const HY = nd == 2 ? MarchShape<2>::HY : MarchShape<3>::HY;
+    int const TX = HX - 2;
+    int const TY = (nd == 3) ? HY - 2 : 1;
+    P.L.ntile[0] = (G.n[0] + TX - 1) / TX;
+    P.L.ntile[1] = (nd == 3) ? (G.n[1] + TY - 1) / TY : 1;
+    int64_t const tiles = (int64_t)P.L.ntile[0] * P.L.ntile[1];
+    int64_t const resident = (int64_t)std::max(blocks_per_cu, 1) * std::max(ncu, 1);
     constexpr int kMinSeg = 4;
-    auto fit = [&](int HX, int HY, int NT, int SEG) {
-        int seg = SEG;
-        shape(HX, HY, NT, seg);
-        while (nd > 1 && nb * (NT / 64) < kFillWaves && seg > kMinSeg) {
-            seg /= 2;
-            shape(HX, HY, NT, seg);
+    int64_t best_cost = -1;
+    for (int seg = std::min(kMinSeg, nm); seg <= nm; ++seg) {
+        int64_t const nb = tiles * ((nm + seg - 1) / seg);
+        int64_t const cost = ((nb + resident - 1) / resident) * (seg + 2);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            P.L.seglen = seg;
+            P.nb = nb;
         }
-    };
-    if (nd == 1) shape(MarchShape<1>::HX, MarchShape<1>::HY, MarchShape<1>::NT, MarchShape<1>::SEG);
-    else if (nd == 2) fit(MarchShape<2>::HX, MarchShape<2>::HY, MarchShape<2>::NT, MarchShape<2>::SEG);
-    else fit(MarchShape<3>::HX, MarchShape<3>::HY, MarchShape<3>::NT, MarchShape<3>::SEG);
-    if (layout) *layout = L;
-    return nb;
+    }
+    return P;
 }
 
-void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, hipStream_t s)
+int march_occupancy(int nd, int geom, int rs, int lim, bool fp32, bool fx)
 {
-    launch_march_any<PerfectGas, 0>(nd, geom, rs, lim, G, eos, A, s);
+    if (fx) return march_occupancy_fx(nd, geom, rs, lim, fp32);
+    int n = 0;
+    if (fp32) march_visit(nd, geom, rs, lim, MarchOccupancy<PerfectGasMixed, 0>{&n});
+    else march_visit(nd, geom, rs, lim, MarchOccupancy<PerfectGas, 0>{&n});
+    return n;
 }
 
-void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGasMixed& eos, const StepArraysF& A, hipStream_t s)
+void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGas& eos, const StepArrays& A,
+                       hipStream_t s)
 {
-    launch_march_any<PerfectGasMixed, 0>(nd, geom, rs, lim, G, eos, A, s);
+    march_visit(nd, geom, rs, lim, MarchLaunch<PerfectGas, 0>{G, P, eos, A, s});
+}
+
+void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGasMixed& eos,
+                       const StepArraysF& A, hipStream_t s)
+{
+    march_visit(nd, geom, rs, lim, MarchLaunch<PerfectGasMixed, 0>{G, P, eos, A, s});
 }
 
 GhostShell make_ghost_shell(const GridDev& G, int nd)

@@ -10,14 +10,24 @@
 
 namespace hclpp {
 
-void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, hipStream_t s)
+void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGas& eos, const StepArrays& A,
+                          hipStream_t s)
 {
-    launch_march_any<PerfectGas, 1>(nd, geom, rs, lim, G, eos, A, s);
+    march_visit(nd, geom, rs, lim, MarchLaunch<PerfectGas, 1>{G, P, eos, A, s});
 }
 
-void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGasMixed& eos, const StepArraysF& A, hipStream_t s)
+void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGasMixed& eos,
+                          const StepArraysF& A, hipStream_t s)
 {
-    launch_march_any<PerfectGasMixed, 1>(nd, geom, rs, lim, G, eos, A, s);
+    march_visit(nd, geom, rs, lim, MarchLaunch<PerfectGasMixed, 1>{G, P, eos, A, s});
+}
+
+int march_occupancy_fx(int nd, int geom, int rs, int lim, bool fp32)
+{
+    int n = 0;
+    if (fp32) march_visit(nd, geom, rs, lim, MarchOccupancy<PerfectGasMixed, 1>{&n});
+    else march_visit(nd, geom, rs, lim, MarchOccupancy<PerfectGas, 1>{&n});
+    return n;
 }
 
 }  // namespace hclpp

@@ -239,6 +239,9 @@ int hclpp_steps_adjoint(hclpp_ctx* ctx, int n, const double* dts, const double* 
  * launches. */
 int hclpp_profile_begin(hclpp_ctx* ctx, int max_steps);
 int hclpp_profile_end(hclpp_ctx* ctx, double* kernel_ms, int* launches);
+/* Launch plan of the fused kernel chosen at context creation: planes per workgroup segment,
+ * workgroups per launch, and workgroups of that kernel resident per CU. */
+int hclpp_fused_plan(hclpp_ctx* ctx, int* seglen, long long* workgroups, int* blocks_per_cu);
 
 #ifdef __cplusplus
 }
