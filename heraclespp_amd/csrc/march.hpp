@@ -299,10 +299,17 @@ __device__ __forceinline__ State<ND, T> dummy_state()
 }
 
 // Minimum waves per SIMD the register allocator must allow.  Measured on MI355X: the 3D
-// FP32-storage kernel gains 23 % at 3 waves (168 VGPRs, 20 B of spills per lane); the FP64
-// kernels lose when forced above their natural 1 (3D) or 2 (2D) waves, because they spill.
-template <class Eos, int ND>
-constexpr int march_min_waves() { return (!std::is_same_v<Eos, PerfectGas> && ND == 3) ? 3 : 1; }
+// FP32-storage kernel gains 23 % at 3 waves (168 VGPRs); the 3D Cartesian FP64 kernel gains 30 %
+// at 2 waves (256 VGPRs, 44 B of spills per lane, with the LDS parking and the shared exchange
+// buffer below so that two workgroups fit a CU's LDS); the spherical 3D FP64 kernel loses 14 %
+// at 2 waves (224 B of spills) and the 2D kernels lose when forced above their natural 2 or 4.
+template <class Eos, int ND, int GEOM>
+constexpr int march_min_waves()
+{
+    if constexpr (ND != 3) return 1;
+    else if constexpr (!std::is_same_v<Eos, PerfectGas>) return 3;
+    else return GEOM == kCartesian ? 2 : 1;
+}
 
 // Load the NFX passive scalars (mass fractions, layout [f][cell]) of one cell.
 template <int NFX, class S>
@@ -316,7 +323,7 @@ __device__ __forceinline__ Frac<NFX, S> load_frac(const StepArraysT<S>& A, const
 
 // NFX: passive scalars carried through the step (0, or 1 for the kernels of step_fused_fx.hip).
 template <class Eos, int ND, int GEOM, int RS, int LIM, int NFX, int HX, int HY, int NT>
-__global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march(GridDev G, Eos eos, StepArraysT<typename Eos::real> A, FusedLayout L)
+__global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step_march(GridDev G, Eos eos, StepArraysT<typename Eos::real> A, FusedLayout L)
 {
     using T = typename Eos::real;
     using X = Frac<NFX, T>;
@@ -333,8 +340,12 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march
     // face states and fluxes have buffers of their own, so no write waits for the readers of
     // the previous plane's buffer beyond the barrier that already separates them.
     __shared__ T s_prim[2][NL][PY][PX];
+    // 3D Cartesian FP64 (two workgroups per CU): the fluxes reuse the face-state buffer, at the
+    // cost of one more barrier per plane, so that the two workgroups fit a CU's LDS.
+    constexpr bool kShare = std::is_same_v<Eos, PerfectGas> && ND == 3 && GEOM == kCartesian;
     __shared__ T s_st[NXCH][NL][HY][HX];
-    __shared__ T s_fl[NXCH][NL][HY][HX];
+    __shared__ T s_fl_own[kShare ? 1 : NXCH][NL][HY][HX];
+    T (*s_fl)[NL][HY][HX] = kShare ? s_st : s_fl_own;
     __shared__ double s_red[NT / 64];
     // Per-thread parking of the values plane h-1 hands to plane h (its right march face state,
     // its partial update and its left march flux, and the scalar's): written at the end of a
@@ -342,8 +353,9 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march
     // slots, so no barrier is involved, and the registers are free across the plane's
     // arithmetic.  Used where it pays (measured): the 3D FP32-storage kernel, whose register
     // budget is set for 3 waves per SIMD (spills go from 40-152 to 0-92 B per lane; RT 512^3
-    // +6 %, spherical Sedov +26 %).  In FP64 and in 2D it gained nothing or lost.
-    constexpr bool kPark = !std::is_same_v<Eos, PerfectGas> && ND == 3;
+    // +6 %, spherical Sedov +26 %), and the 3D Cartesian FP64 kernel run at 2 waves per SIMD.
+    // In spherical FP64 and in 2D it gained nothing or lost.
+    constexpr bool kPark = ND == 3 && (!std::is_same_v<Eos, PerfectGas> || GEOM == kCartesian);
     constexpr int NPARK = kPark ? 3 * NV + 3 * NFX : 1;
     __shared__ T s_park[NPARK][kPark ? NT : 1];
 
@@ -645,11 +657,23 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND>())) void k_step_march
                     FLp[p] = F;
                     XLp[p] = XF;
                     // s_fl was last read before the primitive barrier of this plane
-                    if (in_block) {
+                    if (!kShare && in_block) {
 #pragma unroll
                         for (int v = 0; v < NV; ++v) s_fl[p][v][hy][hx] = F[v];
 #pragma unroll
                         for (int f = 0; f < NFX; ++f) s_fl[p][NV + f][hy][hx] = XF.f[f];
+                    }
+                }
+                if constexpr (kShare) {  // every left state is read before the buffer takes the fluxes
+                    __syncthreads();
+                    if (in_block) {
+#pragma unroll
+                        for (int p = 0; p < NP; ++p) {
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) s_fl[p][v][hy][hx] = FLp[p][v];
+#pragma unroll
+                            for (int f = 0; f < NFX; ++f) s_fl[p][NV + f][hy][hx] = XLp[p].f[f];
+                        }
                     }
                 }
                 __syncthreads();
