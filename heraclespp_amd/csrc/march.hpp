@@ -197,12 +197,13 @@ __device__ __forceinline__ void predict_faces(const GridDev& G, const Eos& eos, 
     for (int dim = 0; dim < ND; ++dim)
 #pragma unroll
         for (int s = 0; s < 2; ++s) fs[s][dim] = prim_to_cons(fp[s][dim], eos);
-    T rho_old[2][ND], mom_old[2][ND][ND];
+    T rho_old[2][ND], mom_old[2][ND][ND], E_face[2][ND];
 #pragma unroll
     for (int dim = 0; dim < ND; ++dim)
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             rho_old[s][dim] = fs[s][dim][0];
+            E_face[s][dim] = fs[s][dim][ND + 1];  // the face's total energy, reused by its flux
 #pragma unroll
             for (int v = 0; v < ND; ++v) mom_old[s][dim][v] = fs[s][dim][1 + v];
         }
@@ -220,8 +221,8 @@ __device__ __forceinline__ void predict_faces(const GridDev& G, const Eos& eos, 
     for (int dim = 0; dim < ND; ++dim) {
         State<ND, T> const& pL = fp[0][dim];
         State<ND, T> const& pR = fp[1][dim];
-        State<ND, T> const fL = flux_of_prim(pL, dim, eos);
-        State<ND, T> const fR = flux_of_prim(pR, dim, eos);
+        State<ND, T> const fL = flux_of_prim_e<ND, T>(pL, dim, E_face[0][dim]);
+        State<ND, T> const fR = flux_of_prim_e<ND, T>(pR, dim, E_face[1][dim]);
         T const dS = T(face_area<ND, GEOM>(G, dim, i, j, k));
         T const dSp = T(face_area<ND, GEOM>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2)));
 #pragma unroll

@@ -180,13 +180,13 @@ __host__ __device__ __forceinline__ State<ND, typename Eos::real> prim_to_cons(c
     return c;
 }
 
-// euler_equations.hpp:72-86: physical flux along `dir` from a primitive state
-template <int ND, class Eos>
-__host__ __device__ __forceinline__ State<ND, typename Eos::real> flux_of_prim(const State<ND, typename Eos::real>& p, int dir, const Eos& eos)
+// euler_equations.hpp:72-86: physical flux along `dir` of a primitive state whose total energy
+// E = kinetic + internal is already known (prim_to_cons forms the same sum for the same state,
+// evol + kinetic, and IEEE addition commutes, so a caller holding both reuses it)
+template <int ND, class T>
+__host__ __device__ __forceinline__ State<ND, T> flux_of_prim_e(const State<ND, T>& p, int dir, T E)
 {
-    using T = typename Eos::real;
     State<ND, T> f;
-    T const E = kinetic_prim(p) + eos.evol_from_pres(p[0], p[ND + 1]);
     T const un = p[1 + dir];
     f[0] = p[0] * un;
 #pragma unroll
@@ -194,6 +194,14 @@ __host__ __device__ __forceinline__ State<ND, typename Eos::real> flux_of_prim(c
     f[1 + dir] += p[ND + 1];
     f[ND + 1] = un * (E + p[ND + 1]);
     return f;
+}
+
+// euler_equations.hpp:72-86: physical flux along `dir` from a primitive state
+template <int ND, class Eos>
+__host__ __device__ __forceinline__ State<ND, typename Eos::real> flux_of_prim(const State<ND, typename Eos::real>& p, int dir, const Eos& eos)
+{
+    using T = typename Eos::real;
+    return flux_of_prim_e<ND, T>(p, dir, kinetic_prim(p) + eos.evol_from_pres(p[0], p[ND + 1]));
 }
 
 // euler_equations.hpp:93-109: physical flux along `dir` from a conservative state
@@ -292,8 +300,6 @@ __host__ __device__ __forceinline__ State<ND, typename Eos::real> riemann_hllc(c
     T const uR = pR[1 + dir];
     T const wsL = fmin(uL - sL, uR - sR);
     T const wsR = fmax(uL + sL, uR + sR);
-    State<ND, T> const fL = flux_of_prim(pL, dir, eos);
-    State<ND, T> const fR = flux_of_prim(pR, dir, eos);
     T const rcL = pL[0] * (wsL - uL);
     T const rcR = pR[0] * (wsR - uR);
     T const ustar = Eos::div(pR[ND + 1] - pL[ND + 1] + (rcL * uL) - (rcR * uR), rcL - rcR);
@@ -302,7 +308,9 @@ __host__ __device__ __forceinline__ State<ND, typename Eos::real> riemann_hllc(c
     T const S = left ? wsL : wsR;
     State<ND, T> const& cs = left ? cL : cR;
     State<ND, T> const& ps = left ? pL : pR;
-    State<ND, T> const& fs = left ? fL : fR;
+    // the reference evaluates both physical fluxes and keeps the upwind one; only that one is
+    // evaluated here (same expressions on the same state, so the same value)
+    State<ND, T> const fs = flux_of_prim(ps, dir, eos);
     T const un = ps[1 + dir];
     if (wsL * wsR > 0) return fs;
     State<ND, T> f;
