@@ -134,3 +134,44 @@ def test_step_parity_passive_scalar_spherical(cuda, nd, n, steps):
     assert sol.run(1e30, steps) == steps
     for nm, gpu in (("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E), ("fx", sol.fx)):
         np.testing.assert_array_equal(gpu.cpu().numpy(), o.state(nm), err_msg=nm)
+
+
+# ---------------------------------------------------------------- launch plans beyond 4-plane segments
+# Grids large enough that the launch plan picks long segments (several per column, a partial last
+# one): the split must not change a bit.
+
+def _plan(sol):
+    import ctypes
+
+    from heraclespp_amd import _lib
+
+    seg, nwg, occ = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()
+    _lib.check(_lib.lib().hclpp_fused_plan(sol.ctx.handle, ctypes.byref(seg), ctypes.byref(nwg), ctypes.byref(occ)))
+    return seg.value, nwg.value, occ.value
+
+
+@pytest.mark.parametrize("nd,n,steps,g", [(2, [2048, 512], 4, (0.0, 0.0, 0.0)), (3, [128, 128, 64], 3, (0.0, 0.0, -0.1))],
+                         ids=["2d_2048x512", "3d_128x128x64"])
+def test_step_parity_long_segments(cuda, nd, n, steps, g):
+    import heraclespp_amd as H
+    from heraclespp_amd.solver import HydroSolver
+
+    bc = ["Periodic"] * (2 * nd)
+    o = orc.Oracle(nd, n, bc=bc, limiter="VanLeer", riemann="HLLC", g=g)
+    rng = np.random.default_rng(11 + nd)
+    rho = 1.0 + 0.3 * rng.random(o.shape3)
+    u = 0.4 * rng.standard_normal((nd,) + o.shape3)
+    P = 1.0 + 0.3 * rng.random(o.shape3)
+    o.init_from_prim(rho, u, P)
+    for _ in range(steps):
+        o.step(0.4, 1e30, steps)
+    grid = H.Grid(nd, n, 2)
+    sol = HydroSolver(grid, H.PerfectGas(1.4, 1.0), reconstruction="VanLeer", riemann_solver="HLLC", bc=bc, cfl=0.4,
+                      gravity=H.UniformGravity(g))
+    seg, nwg, occ = _plan(sol)
+    print(f"\n[parity-plan] nd={nd} n={n}: {seg}-plane segments, {nwg} workgroups, {occ} per CU", flush=True)
+    assert seg > 4 and n[nd - 1] % seg != 0  # several segments per column, the last one partial
+    sol.init_from_prim(rho, u, P)
+    assert sol.run(1e30, steps) == steps
+    for name, gpu in (("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E)):
+        np.testing.assert_array_equal(gpu.cpu().numpy(), o.state(name), err_msg=name)
