@@ -1,0 +1,52 @@
+"""The GPU step against the analytic solutions of the reference's own test problems
+(test/exact_shock_tube.py, test/exact_sedov.py), independent of the oracle:
+
+* Sod (1000 cells, Minmod + HLL, CFL 0.4, t = 0.2): the fused step, FP64 and FP32 storage,
+  within L1(rho) < 2e-3 of the exact Riemann solution (Toro ch. 4; star state pinned against
+  Toro's table in the oracle tests);
+* 3D spherical Sedov (48 x 8 x 16, HLL + VanLeer, E = 1 in the two innermost shells, rho = 1):
+  the radius of the shell-averaged density peak follows the Sedov-Taylor law
+  R = (E t^2 / (alpha rho))^(1/5), alpha(gamma = 1.4) = 0.851072 (Kamm & Timmes 2007), within
+  one radial cell, after 20000 steps."""
+
+import numpy as np
+import pytest
+
+from heraclespp_amd import setups
+from oracle import exact_riemann as X
+from oracle import oracle as orc
+from tests.helpers import make_solver
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_sod_against_exact_solution(cuda, precision):
+    s = setups.sod(1000)
+    sol = make_solver(s, precision=precision)
+    sol.init_from_prim(s.rho, s.u, s.P)
+    sol.run(s.t_end, 1 << 30)
+    assert abs(sol.t - 0.2) < 1e-15
+    x = orc.regular_nodes(1000, 2, 0.0, 1.0)
+    xc = ((x[:-1] + x[1:]) / 2)[2:-2]
+    re, _, _ = X.solve(xc, 0.5, 0.2, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4)
+    l1 = float(np.mean(np.abs(sol.rho.cpu().numpy()[0, 0, 2:-2] - re)))
+    print(f"\n[analytic] sod {precision}: L1(rho) = {l1:.3e}", flush=True)
+    assert l1 < 2e-3
+
+
+def test_sedov_shock_radius(cuda):
+    s = setups.sedov(48, 8, 16, t_end=1.0)
+    sol = make_solver(s)
+    sol.init_from_prim(s.rho, s.u, s.P)
+    steps = sol.run(s.t_end, 20000)
+    assert steps == 20000
+    rho = sol.rho.cpu().numpy()
+    profile = rho[2:-2, 2:-2, 2:-2].mean(axis=(0, 1))
+    x = orc.regular_nodes(48, 2, s.xmin[0], s.xmax[0])
+    xc = ((x[:-1] + x[1:]) / 2)[2:-2]
+    r_peak = xc[int(np.argmax(profile))]
+    r_st = (1.0 * sol.t ** 2 / (0.851072 * 1.0)) ** 0.2
+    dr = x[3] - x[2]
+    print(f"\n[analytic] sedov t={sol.t:.4f}: peak radius {r_peak:.4f}, Sedov-Taylor {r_st:.4f}, cell {dr:.4f}", flush=True)
+    assert abs(r_peak - r_st) <= dr
