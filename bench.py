@@ -23,6 +23,11 @@ calls.  Prints one JSON line:
   cpu_baseline  the CPU oracle (a restatement of the reference's unfused
              kernels, OpenMP) on a bounded sample of the same workload, timed
              on this host's cores
+  mixed_fp32 the same workload with FP32 storage (FP64 pressure recovery, dt and
+             totals), timed the same way after the FP64 run: Gcell/s, kernel time,
+             ratio to FP64, and the relative L1 distance of its density to the FP64
+             density after the same number of steps (the FP64 path is bit-identical
+             to the reference arithmetic)
   adjoint    differentiable-Euler 1024^2 (VanAlbada + HLLC, 400 steps,
              checkpoint every 20): forward, checkpointed backward and reverse-
              kernel cost per step, and central-difference checks of the gradient
@@ -194,6 +199,51 @@ def calibrate_copy(nbytes: int):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
 
 
+def mixed_block(workload: str, steps: int, warmup: int, value64: float, rho64):
+    """The workload again with FP32 storage: throughput (same timing as the FP64 line) and the
+    density's relative L1 distance to the FP64 run after the same number of steps."""
+    import numpy as np
+    import torch
+
+    from heraclespp_amd import _lib
+    from heraclespp_amd.hydro import Grid, PerfectGas, UniformGravity
+    from heraclespp_amd.solver import HydroSolver
+
+    s = _setup(workload, WORKLOADS[workload][0])
+    grid = Grid(s.ndim, s.n, s.ng, s.geom, s.xmin, s.xmax)
+    solver = HydroSolver(grid, PerfectGas(s.gamma, 1.0), reconstruction=s.reconstruction,
+                         riemann_solver=s.riemann_solver, gravity=UniformGravity(s.gravity), bc=s.bc, cfl=s.cfl,
+                         precision="fp32")
+    solver.init_from_prim(s.rho, s.u, s.P)
+    del s.rho, s.u, s.P
+    cells = 1
+    for d in range(s.ndim):
+        cells *= s.n[d]
+    solver.start_clock(1e300, warmup + steps + 1)
+    for _ in range(warmup):
+        solver.step_async()
+    torch.cuda.synchronize()
+    L = _lib.lib()
+    _lib.check(L.hclpp_profile_begin(solver.ctx.handle, steps))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.step_async()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kms, nl = ctypes.c_double(), ctypes.c_int()
+    _lib.check(L.hclpp_profile_end(solver.ctx.handle, ctypes.byref(kms), ctypes.byref(nl)))
+    clk = solver.finish()
+    if clk.iter != warmup + steps:
+        raise RuntimeError(f"expected {warmup + steps} FP32 steps, clock counted {clk.iter}")
+    value = cells * steps / (t1 - t0) / 1e9
+    rho32 = solver.rho.cpu().numpy()[s.interior()]
+    l1 = float(np.sum(np.abs(rho32 - rho64)) / np.sum(np.abs(rho64)))
+    del solver
+    return {"value": round(value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(kms.value / max(nl.value, 1), 5),
+            "ratio_to_fp64": round(value / value64, 3), "density_rel_l1_vs_fp64": float(f"{l1:.3e}"),
+            "steps_compared": warmup + steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +256,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-Euler adjoint measurement")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-storage run of the same workload")
     ap.add_argument("--adjoint-n", type=int, default=1024)
     ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--adjoint-fd-dirs", type=int, default=4)
@@ -309,10 +360,15 @@ def main():
             "valu": valu,
         },
         "cpu_baseline": None,
+        "mixed_fp32": None,
         "adjoint": None,
     }
+    rho64 = solver.rho.cpu().numpy()[s.interior()]
     del solver
     torch.cuda.empty_cache()
+    if a.precision == "fp64" and not a.no_mixed:
+        line["mixed_fp32"] = mixed_block(a.workload, a.steps, a.warmup, value, rho64)
+        torch.cuda.empty_cache()
     if not a.no_adjoint:
         line["adjoint"] = adjoint_block(a.adjoint_n, a.adjoint_steps, 20, a.adjoint_fd_dirs)
         torch.cuda.empty_cache()
