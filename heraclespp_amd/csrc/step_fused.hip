@@ -141,7 +141,7 @@ template <int NV>
 __global__ __launch_bounds__(1024) void k_clock_advance(DevClock* c, const double* __restrict__ partials, int64_t nblocks,
                                                         const double* __restrict__ ghost_partials, int64_t nghost)
 {
-    __shared__ double lds[16];
+    __shared__ double lds[16][NV + 2];  // per-wave results of every reduced value: one LDS round
     if (c->pub.done || c->pub.status) return;
     double inv = reduce_identity<kMax>(), emin = reduce_identity<kMin>();
     double s[NV];
@@ -155,11 +155,31 @@ __global__ __launch_bounds__(1024) void k_clock_advance(DevClock* c, const doubl
         for (int v = 0; v < NV; ++v) s[v] += p[2 + v];
     }
     for (int64_t t = threadIdx.x; t < nghost; t += 1024) inv = reduce_combine<kMax>(inv, ghost_partials[t]);
-    inv = block_reduce<kMax>(inv, lds, threadIdx.x, 1024);
-    emin = block_reduce<kMin>(emin, lds, threadIdx.x, 1024);
+    // all NV + 2 values reduced together: wave shuffles, one barrier, then thread 0 folds the
+    // 16 wave results in wave order (a fixed order: the totals are deterministic)
+    inv = wave_reduce<kMax>(inv);
+    emin = wave_reduce<kMin>(emin);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) s[v] = block_reduce<kSum>(s[v], lds, threadIdx.x, 1024);
+    for (int v = 0; v < NV; ++v) s[v] = wave_reduce<kSum>(s[v]);
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        lds[wave][0] = inv;
+        lds[wave][1] = emin;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lds[wave][2 + v] = s[v];
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
+    inv = reduce_identity<kMax>();
+    emin = reduce_identity<kMin>();
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s[v] = 0;
+    for (int w = 0; w < 16; ++w) {
+        inv = reduce_combine<kMax>(inv, lds[w][0]);
+        emin = reduce_combine<kMin>(emin, lds[w][1]);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s[v] += lds[w][2 + v];
+    }
     c->pub.min_eint = emin;
     if (isnan(emin) || emin < 0) {
         c->pub.status = 3;  // HCLPP_ERR_STATE: the step is discarded
