@@ -263,8 +263,8 @@ def main():
     a = ap.parse_args()
 
     if a.gpus != 1:
-        print("bench.py: this build runs the hydro step on one GPU over the whole domain; "
-              "multi-GPU runs are not part of it (--gpus must be 1).", file=sys.stderr)
+        print("bench.py: the hydro step runs on one GPU over the whole domain; "
+              "multi-GPU runs are not supported (--gpus must be 1).", file=sys.stderr)
         sys.exit(2)
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)

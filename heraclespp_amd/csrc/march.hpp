@@ -169,7 +169,7 @@ __device__ __forceinline__ Frac<NFX, T> zero_frac()
 // scalars of the cell and of the same neighbours, xf their face values after the
 // half step (mass fractions, as fx_rec leaves the predictor).
 template <class Eos, int ND, int GEOM, int LIM, int NFX>
-__device__ __forceinline__ void predict_faces(const GridDev& G, const Eos& eos, typename Eos::real dth, double dth_d, int i, int j, int k,
+__device__ __forceinline__ void predict_faces(const GridDev& G, const Eos& eos, typename Eos::real dth, typename Eos::real dtodv, int i, int j, int k,
                                               const State<ND, typename Eos::real>& p, const State<ND, typename Eos::real> (&nb)[ND][2],
                                               State<ND, typename Eos::real> (&fs)[2][ND], const Frac<NFX, typename Eos::real>& x,
                                               const Frac<NFX, typename Eos::real> (&xnb)[ND][2], Frac<NFX, typename Eos::real> (&xf)[2][ND])
@@ -215,7 +215,6 @@ __device__ __forceinline__ void predict_faces(const GridDev& G, const Eos& eos, 
 #pragma unroll
                 for (int f = 0; f < NFX; ++f) xf[s][dim].f[f] = xo[s][dim].f[f] * rho_old[s][dim];
     }
-    T const dtodv = T(dth_d / cell_volume<ND, GEOM>(G, i, j, k));
     T const x0 = T(G.x[0][i]), x0p = T(G.x[0][i + 1]);
 #pragma unroll
     for (int dim = 0; dim < ND; ++dim) {
@@ -461,6 +460,13 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
     // while plane h is processed; ring cell: plane h+1), so their latency hides behind a plane
     // of arithmetic instead of stalling the workgroup at the LDS barrier.
     State<ND, T> pm, p0, pp, c0, cp, cnext, rnext;
+    // Carried from plane to plane where registers allow (measured): the internal energies of c0
+    // and cp that their pressure recovery forms, and dt / dV of plane h-1 for its completion.
+    // The 3D Cartesian FP64 kernel (2 waves per SIMD) and the 3D spherical FP32 one recompute
+    // them instead: the extra registers spill there.
+    constexpr bool kCarry = !(ND == 3 && (std::is_same_v<Eos, PerfectGas> == (GEOM == kCartesian)));
+    double e0 = 1, ep = 1;
+    T dtodv_prev = 0;
     State<ND, T> fsR_prev, Fm_prev, part_prev;  // in registers, or in s_park when kPark
     T old_rho_prev = 0, old_mom_prev = 0, src_prev = 0;
     // passive scalars: mass fractions of planes h-1, h, h+1 and the prefetch; the ring cell's
@@ -474,7 +480,8 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
     if (col_in_array) {
         pm = prim_at(a, bb, zfirst - 1);
         c0 = load_cons<ND>(A, G, cell_at<ND>(G, a, bb, zfirst));
-        p0 = cons_to_prim(c0, eos);
+        if constexpr (kCarry) p0 = cons_to_prim_e(c0, eos, e0);
+        else p0 = cons_to_prim(c0, eos);
         if (zfirst <= zlast) cnext = load_cons<ND>(A, G, cell_at<ND>(G, a, bb, zfirst + 1));
         if constexpr (NFX > 0) {
             xm = frac_at(a, bb, zfirst - 1);
@@ -518,7 +525,8 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
                 cnext = load_cons<ND>(A, G, cell_at<ND>(G, a, bb, h + 2));
                 if constexpr (NFX > 0) xnext = load_frac<NFX>(A, G, cell_at<ND>(G, a, bb, h + 2));
             }
-            pp = cons_to_prim(cp, eos);
+            if constexpr (kCarry) pp = cons_to_prim_e(cp, eos, ep);
+            else pp = cons_to_prim(cp, eos);
         } else {
             cp = dummy_state<ND, T>();
             pp = dummy_state<ND, T>();
@@ -557,6 +565,7 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
         // ---- predictor of plane h
         State<ND, T> fs[2][ND];
         X xf[2][ND];
+        double qv = 0;
         if (hancock_ok) {
             State<ND, T> nb3[ND][2];
             X xnb[ND][2];
@@ -594,7 +603,16 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
             nb3[MD][1] = pp;
             xnb[MD][0] = xm;
             xnb[MD][1] = xp;
-            predict_faces<Eos, ND, GEOM, LIM, NFX>(G, eos, dth, dth_d, ci, cj, ck, p0, nb3, fs, x0, xnb, xf);
+            // dt / dV once per cell where registers allow (kCarry): the half step's (dt / 2) / dV is
+            // exactly half of it (a power-of-two scaling commutes with rounding in the normal range)
+            T dtodv_h;
+            if constexpr (kCarry) {
+                qv = dt_d / cell_volume<ND, GEOM>(G, ci, cj, ck);
+                dtodv_h = T(qv * 0.5);
+            } else {
+                dtodv_h = T(dth_d / cell_volume<ND, GEOM>(G, ci, cj, ck));
+            }
+            predict_faces<Eos, ND, GEOM, LIM, NFX>(G, eos, dth, dtodv_h, ci, cj, ck, p0, nb3, fs, x0, xnb, xf);
         } else {
 #pragma unroll
             for (int s = 0; s < 2; ++s)
@@ -614,9 +632,10 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
         T dtodv = 0;
         if (out_plane) {
             if (out_col) {
-                dtodv = T(dt_d / cell_volume<ND, GEOM>(G, ci, cj, ck));
-                // min internal energy of the consumed state (internal_energy.cpp:25-40)
-                r_emin = reduce_combine<kMin>(r_emin, state_eint<ND>(eos, c0));
+                dtodv = kCarry ? T(qv) : T(dt_d / cell_volume<ND, GEOM>(G, ci, cj, ck));
+                // min internal energy of the consumed state (internal_energy.cpp:25-40), as its
+                // pressure recovery formed it
+                r_emin = reduce_combine<kMin>(r_emin, kCarry ? e0 : state_eint<ND>(eos, c0));
             }
             if constexpr (ND >= 2) {
                 State<ND, T> FLp[NXCH], FRp[NXCH];
@@ -731,7 +750,7 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
                 int pi, pj, pk;
                 ijk_of<ND>(a, bb, hm, pi, pj, pk);
                 double const vol = cell_volume<ND, GEOM>(G, pi, pj, pk);
-                T const dtodv_m = T(dt_d / vol);
+                T const dtodv_m = kCarry ? dtodv_prev : T(dt_d / vol);
                 T const dS = T(face_area<ND, GEOM>(G, MD, pi, pj, pk));
                 T const dSp = T(face_area<ND, GEOM>(G, MD, ci, cj, ck));
                 if constexpr (kPark) {
@@ -805,6 +824,10 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
         pm = p0;
         p0 = pp;
         c0 = cp;
+        if constexpr (kCarry) {
+            e0 = ep;
+            dtodv_prev = dtodv;
+        }
     }
 
     // ---- workgroup partials

@@ -86,6 +86,13 @@ struct PerfectGas {
         double const evol = c[ND + 1] - kinetic_cons_with(c, div);
         return pres_from_evol(c[0], evol);
     }
+    // the same, also returning the internal energy E - |m|^2 / 2 rho it recovers the pressure from
+    template <int ND>
+    __host__ __device__ __forceinline__ double pressure_of_cons(const State<ND, double>& c, double& evol) const
+    {
+        evol = c[ND + 1] - kinetic_cons_with(c, div);
+        return pres_from_evol(c[0], evol);
+    }
     template <int ND>
     __host__ __device__ __forceinline__ double internal_energy_fp64(const State<ND, double>& c) const
     {
@@ -126,6 +133,12 @@ struct PerfectGasMixed {
     {
         return (float)(gamma_m1 * internal_energy_fp64(c));
     }
+    template <int ND>
+    __host__ __device__ __forceinline__ float pressure_of_cons(const State<ND, float>& c, double& evol) const
+    {
+        evol = internal_energy_fp64(c);
+        return (float)(gamma_m1 * evol);
+    }
 };
 
 // euler_equations.hpp:41-49
@@ -165,6 +178,20 @@ __host__ __device__ __forceinline__ State<ND, typename Eos::real> cons_to_prim(c
 #pragma unroll
     for (int d = 0; d < ND; ++d) p[1 + d] = Eos::div(c[1 + d], c[0]);
     p[ND + 1] = eos.pressure_of_cons(c);
+    return p;
+}
+
+// cons_to_prim that also returns the internal energy of the state (in FP64 for both policies),
+// which the pressure recovery forms anyway (internal_energy.cpp:25-40 computes the same value)
+template <int ND, class Eos>
+__host__ __device__ __forceinline__ State<ND, typename Eos::real> cons_to_prim_e(const State<ND, typename Eos::real>& c, const Eos& eos,
+                                                                                double& evol)
+{
+    State<ND, typename Eos::real> p;
+    p[0] = c[0];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) p[1 + d] = Eos::div(c[1 + d], c[0]);
+    p[ND + 1] = eos.pressure_of_cons(c, evol);
     return p;
 }
 

@@ -240,7 +240,8 @@ class Oracle:
         cfg.pressure_fix = 1 if pressure_fix else 0
         cfg.eps_pf = eps_pf
         self._cfg = cfg
-        h = lib().or_create(ctypes.byref(cfg))
+        self._L = lib()
+        h = self._L.or_create(ctypes.byref(cfg))
         if not h:
             raise RuntimeError(lib().or_last_error().decode())
         self._h = ctypes.c_void_p(h)
@@ -252,8 +253,9 @@ class Oracle:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            lib().or_destroy(h)
+        L = getattr(self, "_L", None)  # held by the object: module globals may be gone at exit
+        if h and L is not None:
+            L.or_destroy(h)
             self._h = None
 
     # ---- ranges (mesh/range.cpp)

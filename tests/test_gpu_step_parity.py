@@ -175,3 +175,16 @@ def test_step_parity_long_segments(cuda, nd, n, steps, g):
     assert sol.run(1e30, steps) == steps
     for name, gpu in (("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E)):
         np.testing.assert_array_equal(gpu.cpu().numpy(), o.state(name), err_msg=name)
+
+
+def test_step_parity_full_size_kelvin_helmholtz(cuda):
+    """The measured workload itself (2D Kelvin-Helmholtz 2048^2, VanLeer + HLLC, CFL 0.4), three
+    steps: bit-identical to the oracle at full size (35-plane segments, 2006 workgroups)."""
+    s = setups.kelvin_helmholtz(2048, seed=0)
+    o = run_oracle(make_oracle(s), s, 3)
+    sol = make_solver(s)
+    sol.init_from_prim(s.rho, s.u, s.P)
+    assert sol.run(s.t_end, 3) == 3
+    assert sol.t == o.t
+    for name, gpu in (("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E)):
+        np.testing.assert_array_equal(gpu.cpu().numpy(), o.state(name), err_msg=name)
