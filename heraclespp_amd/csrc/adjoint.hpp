@@ -10,6 +10,11 @@
 // sign test and Minmod selection, the min/max wave speeds, HLL's clipping of
 // the wave speeds at zero, and HLLC's choice of star state (sign of u*) and of
 // the supersonic branch (wsL * wsR > 0).
+//
+// The forward values that decide a branch (slopes, wave speeds, u*) are recomputed with the
+// forward's own expressions.  The derivative formulas themselves are free of that constraint:
+// they divide once per divisor and multiply by the reciprocal (a division is an 11-instruction
+// sequence around a quarter-rate reciprocal on gfx950), which moves results by rounding only.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,19 +42,20 @@ template <int ND>
 __host__ __device__ __forceinline__ void prim_vjp(const S<ND>& c, double gamma, const S<ND>& lp, S<ND>& lc, double& lg)
 {
     double const rho = c[0];
+    double const ir = 1 / rho;
     double const gm1 = gamma - 1;
     double u2 = 0, ulu = 0;
     double u[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-        u[d] = c[1 + d] / rho;
+        u[d] = c[1 + d] * ir;
         u2 += u[d] * u[d];
         ulu += lp[1 + d] * u[d];
     }
     double const lP = lp[ND + 1];
-    lc[0] += lp[0] - ulu / rho + lP * gm1 * 0.5 * u2;
+    lc[0] += lp[0] - ulu * ir + lP * gm1 * 0.5 * u2;
 #pragma unroll
-    for (int d = 0; d < ND; ++d) lc[1 + d] += lp[1 + d] / rho - lP * gm1 * u[d];
+    for (int d = 0; d < ND; ++d) lc[1 + d] += lp[1 + d] * ir - lP * gm1 * u[d];
     lc[ND + 1] += lP * gm1;
     lg += lP * (c[ND + 1] - 0.5 * rho * u2);
 }
@@ -66,12 +72,13 @@ __host__ __device__ __forceinline__ void cons_vjp(const S<ND>& p, double gamma, 
         u2 += p[1 + d] * p[1 + d];
         lmu += lc[1 + d] * p[1 + d];
     }
+    double const igm1 = 1 / gm1;
     double const lE = lc[ND + 1];
     lp[0] += lc[0] + lmu + lE * 0.5 * u2;
 #pragma unroll
     for (int d = 0; d < ND; ++d) lp[1 + d] += lc[1 + d] * rho + lE * rho * p[1 + d];
-    lp[ND + 1] += lE / gm1;
-    lg += -lE * p[ND + 1] / (gm1 * gm1);
+    lp[ND + 1] += lE * igm1;
+    lg += -lE * p[ND + 1] * igm1 * igm1;
 }
 
 // physical flux along n from a primitive state
@@ -87,7 +94,8 @@ __host__ __device__ __forceinline__ void flux_vjp(const S<ND>& p, int n, double 
         u2 += p[1 + k] * p[1 + k];
         au += lf[1 + k] * p[1 + k];
     }
-    double const E = P / gm1 + 0.5 * rho * u2;
+    double const igm1 = 1 / gm1;
+    double const E = P * igm1 + 0.5 * rho * u2;
     double const aE = lf[ND + 1];
     lp[0] += lf[0] * un + un * au + aE * un * 0.5 * u2;
 #pragma unroll
@@ -96,16 +104,17 @@ __host__ __device__ __forceinline__ void flux_vjp(const S<ND>& p, int n, double 
         if (j == n) g += lf[0] * rho + rho * au + aE * (E + P);
         lp[1 + j] += g;
     }
-    lp[ND + 1] += lf[1 + n] + aE * un * (1 / gm1 + 1);
-    lg += -aE * un * P / (gm1 * gm1);
+    lp[ND + 1] += lf[1 + n] + aE * un * (igm1 + 1);
+    lg += -aE * un * P * igm1 * igm1;
 }
 
 // c = sqrt(gamma P / rho)
 __host__ __device__ __forceinline__ void sound_vjp(double rho, double P, double gamma, double c, double lc_, double& lrho, double& lP, double& lg)
 {
-    lrho += -lc_ * c / (2 * rho);
-    lP += lc_ * gamma / (2 * rho * c);
-    lg += lc_ * P / (2 * rho * c);
+    double const i2rc = 1 / (2 * rho * c);  // c / (2 rho) = c^2 / (2 rho c)
+    lrho += -lc_ * c * c * i2rc;
+    lP += lc_ * gamma * i2rc;
+    lg += lc_ * P * i2rc;
 }
 
 // slope limiter: derivative of the limited slope w.r.t. (diffR, diffL) on the forward branch
@@ -120,8 +129,9 @@ __host__ __device__ __forceinline__ void limiter_grad(double dR, double dL, doub
         if (!(dL * dR > 0)) return;
         if constexpr (LIM == kVanLeer) {  // 2 dR dL / (dR + dL)
             double const s = dR + dL;
-            gR = 2 * dL * dL / (s * s);
-            gL = 2 * dR * dR / (s * s);
+            double const is2 = 1 / (s * s);
+            gR = 2 * dL * dL * is2;
+            gL = 2 * dR * dR * is2;
         } else if constexpr (LIM == kMinmod) {  // dR if r <= 1 else dL  (fmin(1, r) branch)
             double const r = dR / dL;
             if (r < 1.) gR = 1;
@@ -130,8 +140,9 @@ __host__ __device__ __forceinline__ void limiter_grad(double dR, double dL, doub
         } else {  // VanAlbada: dR dL (dR + dL) / (dR^2 + dL^2)
             double const N = dR * dL * (dR + dL);
             double const D = dR * dR + dL * dL;
-            gR = ((2 * dR * dL + dL * dL) * D - N * 2 * dR) / (D * D);
-            gL = ((2 * dR * dL + dR * dR) * D - N * 2 * dL) / (D * D);
+            double const iD2 = 1 / (D * D);
+            gR = ((2 * dR * dL + dL * dL) * D - N * 2 * dR) * iD2;
+            gL = ((2 * dR * dL + dR * dR) * D - N * 2 * dL) * iD2;
         }
     }
 }
@@ -150,14 +161,15 @@ __host__ __device__ __forceinline__ void hll_vjp(const S<ND>& cL, const S<ND>& c
     double const nL = fmin(wsL, 0.), pRw = fmax(wsR, 0.);
     S<ND> const fL = flux_of_prim(pL, n, eos), fR = flux_of_prim(pR, n, eos);
     double const D = pRw - nL;
+    double const iD = 1 / D;
     S<ND> lfL = zero<ND>(), lfR = zero<ND>();
     double lpR = 0, lnL = 0, lD = 0;
 #pragma unroll
     for (int v = 0; v < ND + 2; ++v) {
         double const dU = cR[v] - cL[v];
-        double const F = ((pRw * fL[v]) - (nL * fR[v]) + (nL * pRw * dU)) / D;
-        double const lN = lF[v] / D;
-        lD -= lF[v] * F / D;
+        double const F = ((pRw * fL[v]) - (nL * fR[v]) + (nL * pRw * dU)) * iD;
+        double const lN = lF[v] * iD;
+        lD -= lF[v] * F * iD;
         lfL[v] += lN * pRw;
         lfR[v] -= lN * nL;
         lcR[v] += lN * nL * pRw;
@@ -213,9 +225,10 @@ __host__ __device__ __forceinline__ void hllc_vjp(const S<ND>& cL, const S<ND>& 
         lfs = lF;  // F = flux of the upwind state
     } else {
         double const A = Sw - un, B = Sw - ustar;
-        double const rs = cs[0] * A / B;
+        double const iB = 1 / B;
+        double const rs = cs[0] * A * iB;
         double const C = (ustar * cs[0] * A) + ps[ND + 1];
-        double const Es = ((A / B) * cs[ND + 1]) + (((ustar - un) / B) * C);
+        double const Es = ((A * iB) * cs[ND + 1]) + (((ustar - un) * iB) * C);
         // f0 = fs0 + S (rho* - rho), fE = fsE + S (E* - E)
         lfs[0] = lF[0];
         lfs[ND + 1] = lF[ND + 1];
@@ -236,20 +249,20 @@ __host__ __device__ __forceinline__ void hllc_vjp(const S<ND>& cL, const S<ND>& 
         double const lEs = lF[ND + 1] * Sw;
         lcs[ND + 1] -= lF[ND + 1] * Sw;
         // E* = (A/B) E + ((u* - un)/B) C, C = u* rho A + P, A = S - un, B = S - u*
-        double lA = lEs * cs[ND + 1] / B;
-        double lB = -lEs * Es / B;
-        lcs[ND + 1] += lEs * A / B;
-        lustar += lEs * C / B;
-        double lun = -lEs * C / B;
-        double const lC = lEs * (ustar - un) / B;
+        double lA = lEs * cs[ND + 1] * iB;
+        double lB = -lEs * Es * iB;
+        lcs[ND + 1] += lEs * A * iB;
+        lustar += lEs * C * iB;
+        double lun = -lEs * C * iB;
+        double const lC = lEs * (ustar - un) * iB;
         lustar += lC * cs[0] * A;
         lcs[0] += lC * ustar * A;
         lA += lC * ustar * cs[0];
         lps[ND + 1] += lC;
         // rho* = rho A / B
-        lcs[0] += lrs * A / B;
-        lA += lrs * cs[0] / B;
-        lB -= lrs * rs / B;
+        lcs[0] += lrs * A * iB;
+        lA += lrs * cs[0] * iB;
+        lB -= lrs * rs * iB;
         lS += lA + lB;
         lun -= lA;
         lustar -= lB;
@@ -265,8 +278,9 @@ __host__ __device__ __forceinline__ void hllc_vjp(const S<ND>& cL, const S<ND>& 
     lpL[1 + n] -= 0.5 * lpstar * rcL;
     lpR[1 + n] -= 0.5 * lpstar * rcR;
     // u* = num / den
-    double const lnum = lustar / den;
-    double const lden = -lustar * ustar / den;
+    double const iden = 1 / den;
+    double const lnum = lustar * iden;
+    double const lden = -lustar * ustar * iden;
     lpR[ND + 1] += lnum;
     lpL[ND + 1] -= lnum;
     lrcL += lnum * uL + lden;

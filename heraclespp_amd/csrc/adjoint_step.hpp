@@ -247,14 +247,16 @@ __host__ __device__ __forceinline__ void slope_adjoint(const GridDev& G, const d
     double const dl = G.dx[d][c[d]];
     double const hp = (dl + G.dx[d][c[d] + 1]) / 2;
     double const hm = (G.dx[d][c[d] - 1] + dl) / 2;
+    double const ihp = 1 / hp, ihm = 1 / hm;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         double gR, gL;
+        // the forward's slope arguments (they decide the limiter branch): its own divisions
         limiter_grad<LIM>((pp[v] - p[v]) / hp, (p[v] - pm[v]) / hm, gR, gL);
         double const ls = (dl / 2) * (l1[v] - l0[v]);
         own[v] = l0[v] + l1[v];
-        ra[v] = ls * gR / hp;
-        rb[v] = ls * gL / hm;
+        ra[v] = ls * gR * ihp;
+        rb[v] = ls * gL * ihm;
     }
 }
 
