@@ -13,8 +13,9 @@
 //
 // The forward values that decide a branch (slopes, wave speeds, u*) are recomputed with the
 // forward's own expressions.  The derivative formulas themselves are free of that constraint:
-// they divide once per divisor and multiply by the reciprocal (a division is an 11-instruction
-// sequence around a quarter-rate reciprocal on gfx950), which moves results by rounding only.
+// they divide once per divisor and multiply by the reciprocal (each division is an 8-instruction
+// sequence around a quarter-rate reciprocal, physics.hpp fdiv), which moves results by rounding
+// only.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,7 +43,7 @@ template <int ND>
 __host__ __device__ __forceinline__ void prim_vjp(const S<ND>& c, double gamma, const S<ND>& lp, S<ND>& lc, double& lg)
 {
     double const rho = c[0];
-    double const ir = 1 / rho;
+    double const ir = fdiv(1, rho);
     double const gm1 = gamma - 1;
     double u2 = 0, ulu = 0;
     double u[ND];
@@ -72,7 +73,7 @@ __host__ __device__ __forceinline__ void cons_vjp(const S<ND>& p, double gamma, 
         u2 += p[1 + d] * p[1 + d];
         lmu += lc[1 + d] * p[1 + d];
     }
-    double const igm1 = 1 / gm1;
+    double const igm1 = fdiv(1, gm1);
     double const lE = lc[ND + 1];
     lp[0] += lc[0] + lmu + lE * 0.5 * u2;
 #pragma unroll
@@ -94,7 +95,7 @@ __host__ __device__ __forceinline__ void flux_vjp(const S<ND>& p, int n, double 
         u2 += p[1 + k] * p[1 + k];
         au += lf[1 + k] * p[1 + k];
     }
-    double const igm1 = 1 / gm1;
+    double const igm1 = fdiv(1, gm1);
     double const E = P * igm1 + 0.5 * rho * u2;
     double const aE = lf[ND + 1];
     lp[0] += lf[0] * un + un * au + aE * un * 0.5 * u2;
@@ -129,18 +130,18 @@ __host__ __device__ __forceinline__ void limiter_grad(double dR, double dL, doub
         if (!(dL * dR > 0)) return;
         if constexpr (LIM == kVanLeer) {  // 2 dR dL / (dR + dL)
             double const s = dR + dL;
-            double const is2 = 1 / (s * s);
+            double const is2 = fdiv(1, s * s);
             gR = 2 * dL * dL * is2;
             gL = 2 * dR * dR * is2;
         } else if constexpr (LIM == kMinmod) {  // dR if r <= 1 else dL  (fmin(1, r) branch)
-            double const r = dR / dL;
+            double const r = fdiv(dR, dL);
             if (r < 1.) gR = 1;
             else if (r > 1.) gL = 1;
             else { gR = 0.5; gL = 0.5; }
         } else {  // VanAlbada: dR dL (dR + dL) / (dR^2 + dL^2)
             double const N = dR * dL * (dR + dL);
             double const D = dR * dR + dL * dL;
-            double const iD2 = 1 / (D * D);
+            double const iD2 = fdiv(1, D * D);
             gR = ((2 * dR * dL + dL * dL) * D - N * 2 * dR) * iD2;
             gL = ((2 * dR * dL + dR * dR) * D - N * 2 * dL) * iD2;
         }
@@ -161,7 +162,7 @@ __host__ __device__ __forceinline__ void hll_vjp(const S<ND>& cL, const S<ND>& c
     double const nL = fmin(wsL, 0.), pRw = fmax(wsR, 0.);
     S<ND> const fL = flux_of_prim(pL, n, eos), fR = flux_of_prim(pR, n, eos);
     double const D = pRw - nL;
-    double const iD = 1 / D;
+    double const iD = fdiv(1, D);
     S<ND> lfL = zero<ND>(), lfR = zero<ND>();
     double lpR = 0, lnL = 0, lD = 0;
 #pragma unroll
@@ -207,7 +208,7 @@ __host__ __device__ __forceinline__ void hllc_vjp(const S<ND>& cL, const S<ND>& 
     double const wsL = fmin(aL, bL), wsR = fmax(aR, bR);
     double const rcL = pL[0] * (wsL - uL), rcR = pR[0] * (wsR - uR);
     double const num = PR - PL + (rcL * uL) - (rcR * uR), den = rcL - rcR;
-    double const ustar = num / den;
+    double const ustar = fdiv(num, den);
     bool const left = ustar > 0;
     // upwind (star-side) state, selected element-wise
     S<ND> cs, ps;
@@ -225,7 +226,7 @@ __host__ __device__ __forceinline__ void hllc_vjp(const S<ND>& cL, const S<ND>& 
         lfs = lF;  // F = flux of the upwind state
     } else {
         double const A = Sw - un, B = Sw - ustar;
-        double const iB = 1 / B;
+        double const iB = fdiv(1, B);
         double const rs = cs[0] * A * iB;
         double const C = (ustar * cs[0] * A) + ps[ND + 1];
         double const Es = ((A * iB) * cs[ND + 1]) + (((ustar - un) * iB) * C);
@@ -278,7 +279,7 @@ __host__ __device__ __forceinline__ void hllc_vjp(const S<ND>& cL, const S<ND>& 
     lpL[1 + n] -= 0.5 * lpstar * rcL;
     lpR[1 + n] -= 0.5 * lpstar * rcR;
     // u* = num / den
-    double const iden = 1 / den;
+    double const iden = fdiv(1, den);
     double const lnum = lustar * iden;
     double const lden = -lustar * ustar * iden;
     lpR[ND + 1] += lnum;

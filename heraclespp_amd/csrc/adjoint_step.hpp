@@ -114,7 +114,7 @@ __host__ __device__ __forceinline__ void face_prims(const GridDev& G, const doub
         double const dl_p = G.dx[d][c[d] + 1];
 #pragma unroll
         for (int v = 0; v < ND + 2; ++v) {
-            double const slope = limited_slope_t<LIM>((pp[v] - p[v]) / ((dl + dl_p) / 2), (p[v] - pm[v]) / ((dl_m + dl) / 2));
+            double const slope = limited_slope_t<LIM>(fdiv(pp[v] - p[v], (dl + dl_p) / 2), fdiv(p[v] - pm[v], (dl_m + dl) / 2));
             pf[0][d][v] = p[v] - ((dl / 2) * slope);
             pf[1][d][v] = p[v] + ((dl / 2) * slope);
         }
@@ -159,7 +159,7 @@ __host__ __device__ __forceinline__ void cell_faces(const GridDev& G, const Perf
     for (int dim = 0; dim < ND; ++dim) {
         S<ND> const fL = flux_of_prim(pf[0][dim], dim, eos);
         S<ND> const fR = flux_of_prim(pf[1][dim], dim, eos);
-        double const dtodv = dth / cell_volume<ND, kCartesian>(G, i, j, k);
+        double const dtodv = fdiv(dth, cell_volume<ND, kCartesian>(G, i, j, k));
         double const dS = face_area<ND, kCartesian>(G, dim, i, j, k);
         double const dSp = face_area<ND, kCartesian>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2));
 #pragma unroll
@@ -200,14 +200,14 @@ __host__ __device__ __forceinline__ double face_flux(const GridDev& G, const Per
     S<ND> lF = zero<ND>();
     // godunov_scheme.hpp:110-292: U'(c) += dt/dV F dS;  U'(c - e_d) -= dt/dV F dS+
     if (inside(G, d, c[d])) {
-        double const w = (A.dt / cell_volume<ND, kCartesian>(G, i, j, k)) * face_area<ND, kCartesian>(G, d, i, j, k);
+        double const w = fdiv(A.dt, cell_volume<ND, kCartesian>(G, i, j, k)) * face_area<ND, kCartesian>(G, d, i, j, k);
         S<ND> const l = load_cons<ND>(A.lrho_new, A.lrhou_new, A.lE_new, id, pl);
 #pragma unroll
         for (int v = 0; v < NV; ++v) lF[v] += w * l[v];
     }
     if (inside(G, d, c[d] - 1)) {
         int const im = i - (d == 0), jm = j - (d == 1), km = k - (d == 2);
-        double const w = (A.dt / cell_volume<ND, kCartesian>(G, im, jm, km)) * face_area<ND, kCartesian>(G, d, i, j, k);
+        double const w = fdiv(A.dt, cell_volume<ND, kCartesian>(G, im, jm, km)) * face_area<ND, kCartesian>(G, d, i, j, k);
         S<ND> const l = load_cons<ND>(A.lrho_new, A.lrhou_new, A.lE_new, idm, pl);
 #pragma unroll
         for (int v = 0; v < NV; ++v) lF[v] -= w * l[v];
@@ -247,12 +247,12 @@ __host__ __device__ __forceinline__ void slope_adjoint(const GridDev& G, const d
     double const dl = G.dx[d][c[d]];
     double const hp = (dl + G.dx[d][c[d] + 1]) / 2;
     double const hm = (G.dx[d][c[d] - 1] + dl) / 2;
-    double const ihp = 1 / hp, ihm = 1 / hm;
+    double const ihp = fdiv(1, hp), ihm = fdiv(1, hm);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         double gR, gL;
         // the forward's slope arguments (they decide the limiter branch): its own divisions
-        limiter_grad<LIM>((pp[v] - p[v]) / hp, (p[v] - pm[v]) / hm, gR, gL);
+        limiter_grad<LIM>(fdiv(pp[v] - p[v], hp), fdiv(p[v] - pm[v], hm), gR, gL);
         double const ls = (dl / 2) * (l1[v] - l0[v]);
         own[v] = l0[v] + l1[v];
         ra[v] = ls * gR * ihp;
@@ -306,7 +306,7 @@ __host__ __device__ __forceinline__ double cell_hancock(const GridDev& G, const 
                 }
                 cons_vjp(pf[s][pos], eos.gamma, lc, lpf[s][pos], lg);
             }
-        double const dtodv = dth / cell_volume<ND, kCartesian>(G, i, j, k);
+        double const dtodv = fdiv(dth, cell_volume<ND, kCartesian>(G, i, j, k));
 #pragma unroll
         for (int dim = 0; dim < ND; ++dim) {
             double const dS = face_area<ND, kCartesian>(G, dim, i, j, k);

@@ -107,7 +107,7 @@ __device__ __forceinline__ double cfl_rate(const GridDev& G, int i, int j, int k
     }
     double rate = 0;
 #pragma unroll
-    for (int d = 0; d < ND; ++d) rate += (fabs(u[d]) + c) / dxg[d];
+    for (int d = 0; d < ND; ++d) rate += fdiv(fabs(u[d]) + c, dxg[d]);
     return rate;
 }
 

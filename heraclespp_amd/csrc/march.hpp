@@ -90,9 +90,9 @@ __device__ __forceinline__ double state_rate(const GridDev& G, const Eos& eos, i
         double const rho = c[0];
         double u[ND];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) u[d] = (double)c[1 + d] / rho;
+        for (int d = 0; d < ND; ++d) u[d] = fdiv((double)c[1 + d], rho);
         double const P = eos.gamma_m1 * eos.internal_energy_fp64(c);
-        return cfl_rate<ND, GEOM>(G, i, j, k, u, sqrt(eos.gamma_d * P / rho));
+        return cfl_rate<ND, GEOM>(G, i, j, k, u, sqrt(fdiv(eos.gamma_d * P, rho)));
     }
 }
 
@@ -607,10 +607,10 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
             // exactly half of it (a power-of-two scaling commutes with rounding in the normal range)
             T dtodv_h;
             if constexpr (kCarry) {
-                qv = dt_d / cell_volume<ND, GEOM>(G, ci, cj, ck);
+                qv = fdiv(dt_d, cell_volume<ND, GEOM>(G, ci, cj, ck));
                 dtodv_h = T(qv * 0.5);
             } else {
-                dtodv_h = T(dth_d / cell_volume<ND, GEOM>(G, ci, cj, ck));
+                dtodv_h = T(fdiv(dth_d, cell_volume<ND, GEOM>(G, ci, cj, ck)));
             }
             predict_faces<Eos, ND, GEOM, LIM, NFX>(G, eos, dth, dtodv_h, ci, cj, ck, p0, nb3, fs, x0, xnb, xf);
         } else {
@@ -632,7 +632,7 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
         T dtodv = 0;
         if (out_plane) {
             if (out_col) {
-                dtodv = kCarry ? T(qv) : T(dt_d / cell_volume<ND, GEOM>(G, ci, cj, ck));
+                dtodv = kCarry ? T(qv) : T(fdiv(dt_d, cell_volume<ND, GEOM>(G, ci, cj, ck)));
                 // min internal energy of the consumed state (internal_energy.cpp:25-40), as its
                 // pressure recovery formed it
                 r_emin = reduce_combine<kMin>(r_emin, kCarry ? e0 : state_eint<ND>(eos, c0));
@@ -750,7 +750,7 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
                 int pi, pj, pk;
                 ijk_of<ND>(a, bb, hm, pi, pj, pk);
                 double const vol = cell_volume<ND, GEOM>(G, pi, pj, pk);
-                T const dtodv_m = kCarry ? dtodv_prev : T(dt_d / vol);
+                T const dtodv_m = kCarry ? dtodv_prev : T(fdiv(dt_d, vol));
                 T const dS = T(face_area<ND, GEOM>(G, MD, pi, pj, pk));
                 T const dSp = T(face_area<ND, GEOM>(G, MD, ci, cj, ck));
                 if constexpr (kPark) {

@@ -11,9 +11,10 @@
 //  * PerfectGasMixed (FP32): state and fluxes in FP32 with hardware reciprocal
 //    divisions, while the pressure recovery P = (gamma-1)(E - |m|^2 / 2 rho),
 //    which cancels at high Mach number, is evaluated in FP64.
-// Division is always written through Eos::div so the FP64 policy keeps the
-// correctly-rounded quotient and the FP32 policy uses v_rcp_f32.  Halving is
-// a multiplication by 0.5, which is exact in IEEE arithmetic.
+// Division is always written through Eos::div (or fdiv below) so the FP64
+// policy keeps the correctly-rounded quotient and the FP32 policy uses
+// v_rcp_f32.  Halving is a multiplication by 0.5, which is exact in IEEE
+// arithmetic.
 // Comments cite the reference file:line each function follows.
 #pragma once
 
@@ -28,6 +29,36 @@ constexpr double kKb = 1.380649E-23;
 constexpr double kMp = 1.672649E-27;
 constexpr double kGrav = 6.6743015E-11;
 constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// FP64 division a / b.  On the device this is the instruction sequence hipcc emits for an IEEE
+// division on gfx950 (v_rcp_f64, two Newton steps on the reciprocal, the quotient, one
+// remainder correction, v_div_fixup_f64) without its two v_div_scale_f64 and with
+// v_div_fmas_f64 replaced by the plain fma it reduces to.  By the ISA's definition of
+// v_div_scale_f64, it returns its operand with VCC clear (so v_div_fmas_f64 is that fma) when
+//   * a and b are finite and normal, |a| >= 2^-969, |b| <= 2^1022, a / b is normal and
+//     exponent(a) - exponent(b) < 768,
+// so there the result is bit-identical to a / b.  v_div_fixup_f64 stays: it sets the sign of
+// every quotient and returns what IEEE division gives for zero and infinite operands and NaNs.
+// Outside that range (finite non-zero operands with a numerator below 2^-969, a subnormal
+// divisor, reciprocal or quotient, or an exponent gap of 768 or more) the result may differ
+// from a / b.  Two instructions fewer per division, and the reciprocal part depends on b
+// alone, so the compiler computes it once for every division sharing a divisor.  The host
+// build divides.
+__host__ __device__ __forceinline__ double fdiv(double a, double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    double const q = a * y;
+    double const r = __builtin_fma(-b, q, a);
+    return __builtin_amdgcn_div_fixup(__builtin_fma(r, y, q), b, a);
+#else
+    return a / b;
+#endif
+}
 
 enum Limiter { kConstant = 0, kVanLeer = 1, kMinmod = 2, kVanAlbada = 3 };
 enum Solver { kHLL = 0, kHLLC = 1, kLowMach = 2 };
@@ -58,27 +89,27 @@ struct PerfectGas {
     double gamma_m1;
     double mmw;
 
-    __host__ __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+    __host__ __device__ static __forceinline__ double div(double a, double b) { return fdiv(a, b); }
     // perfect_gas.hpp:69-72
-    __host__ __device__ __forceinline__ double temp_from_pres(double rho, double P) const { return P * mmw * kMp / (rho * kKb); }
+    __host__ __device__ __forceinline__ double temp_from_pres(double rho, double P) const { return fdiv(P * mmw * kMp, rho * kKb); }
     // perfect_gas.hpp:52-56
-    __host__ __device__ __forceinline__ double evol_from_temp(double rho, double T) const { return rho * kKb * T / (mmw * kMp * gamma_m1); }
+    __host__ __device__ __forceinline__ double evol_from_temp(double rho, double T) const { return fdiv(rho * kKb * T, mmw * kMp * gamma_m1); }
     // perfect_gas.hpp:46-50
     __host__ __device__ __forceinline__ double evol_from_pres(double rho, double P) const { return evol_from_temp(rho, temp_from_pres(rho, P)); }
     // perfect_gas.hpp:74-78
     __host__ __device__ __forceinline__ double temp_from_evol(double rho, double evol) const
     {
-        return evol * mmw * kMp * gamma_m1 / (rho * kKb);
+        return fdiv(evol * mmw * kMp * gamma_m1, rho * kKb);
     }
     // perfect_gas.hpp:64-67
-    __host__ __device__ __forceinline__ double pres_from_temp(double rho, double T) const { return rho * kKb * T / (mmw * kMp); }
+    __host__ __device__ __forceinline__ double pres_from_temp(double rho, double T) const { return fdiv(rho * kKb * T, mmw * kMp); }
     // perfect_gas.hpp:58-62
     __host__ __device__ __forceinline__ double pres_from_evol(double rho, double evol) const
     {
         return pres_from_temp(rho, temp_from_evol(rho, evol));
     }
     // perfect_gas.hpp:80-83
-    __host__ __device__ __forceinline__ double sound(double rho, double P) const { return sqrt(gamma * P / rho); }
+    __host__ __device__ __forceinline__ double sound(double rho, double P) const { return sqrt(fdiv(gamma * P, rho)); }
     // euler_equations.hpp:61-65, :111-122: pressure of a conservative state
     template <int ND>
     __host__ __device__ __forceinline__ double pressure_of_cons(const State<ND, double>& c) const
@@ -126,7 +157,7 @@ struct PerfectGasMixed {
         double n = 0;
 #pragma unroll
         for (int d = 0; d < ND; ++d) n += (double)c[1 + d] * (double)c[1 + d];
-        return (double)c[ND + 1] - n / (double)c[0] * 0.5;
+        return (double)c[ND + 1] - fdiv(n, (double)c[0]) * 0.5;
     }
     template <int ND>
     __host__ __device__ __forceinline__ float pressure_of_cons(const State<ND, float>& c) const

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void k_face_reconstruction(GridDev G, Range3 r
         double const dl_p = G.dx[dim][a + 1];
         double const vp = var[flat(G, i + di, j + dj, k + dk)];
         double const vm = var[flat(G, i - di, j - dj, k - dk)];
-        double const slope = limited_slope_t<LIM>((vp - v) / ((dl + dl_p) / 2), (v - vm) / ((dl_m + dl) / 2));
+        double const slope = limited_slope_t<LIM>(fdiv(vp - v, (dl + dl_p) / 2), fdiv(v - vm, (dl_m + dl) / 2));
         rec[id + G.plane * (0 + 2 * dim)] = v - ((dl / 2) * slope);
         rec[id + G.plane * (1 + 2 * dim)] = v + ((dl / 2) * slope);
     }
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_hancock(GridDev G, Range3 r, PerfectGas
         pR[ND + 1] = P_rec[id + pl * (1 + 2 * dim)];
         State<ND> const fL = flux_of_prim(pL, dim, eos);
         State<ND> const fR = flux_of_prim(pR, dim, eos);
-        double const dtodv = dt / cell_volume<ND, GEOM>(G, i, j, k);
+        double const dtodv = fdiv(dt, cell_volume<ND, GEOM>(G, i, j, k));
         double const dS = face_area<ND, GEOM>(G, dim, i, j, k);
         double const dSp = face_area<ND, GEOM>(G, dim, ip, jp, kp);
 #pragma unroll
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_godunov(GridDev G, Range3 r, PerfectGas
         State<ND> const vR = face(id, 1, dim);
         State<ND> const FL = riemann<RS>(face(idm, 1, dim), vL, dim, eos);
         State<ND> const FR = riemann<RS>(vR, face(idp, 0, dim), dim, eos);
-        double const dtodv = dt / cell_volume<ND, GEOM>(G, i, j, k);
+        double const dtodv = fdiv(dt, cell_volume<ND, GEOM>(G, i, j, k));
         double const dS = face_area<ND, GEOM>(G, dim, i, j, k);
         double const dSp = face_area<ND, GEOM>(G, dim, i + di, j + dj, k + dk);
 #pragma unroll
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void k_godunov(GridDev G, Range3 r, PerfectGas
     for (int d = 0; d < ND; ++d) rhou_new[id + pl * d] = un[1 + d];
     E_new[id] = un[ND + 1];
     for (int f = 0; f < nfx; ++f) {
-        double const y = fx_new[id + pl * f] / un[0];
+        double const y = fdiv(fx_new[id + pl * f], un[0]);
         fx_new[id + pl * f] = y < 0. ? 0. : (y > 1. ? 1. : y);  // Kokkos::clamp(y, 0, 1)
     }
 }

@@ -1,0 +1,25 @@
+"""The FP64 division of the hydro kernels (physics.hpp ``fdiv``: the hardware division sequence
+without its range-scaling instructions) must equal the IEEE quotient bit for bit on its stated
+range.  ``tests/cpp/div_check`` divides 2^24 random and adversarial operand pairs on the GPU and
+compares every in-range quotient with the host's IEEE division; out-of-range pairs are only
+reported."""
+
+import os
+import re
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "div_check")
+
+
+def test_kernel_division_is_ieee_on_its_range(cuda):
+    assert os.path.exists(BIN), "build it with __graft_entry__.build() (make -C tests/cpp)"
+    print("[division] running tests/cpp/div_check", flush=True)
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"in-range: (\d+) pairs, (\d+) mismatches", r.stdout)
+    assert m and int(m.group(1)) > 15_000_000 and int(m.group(2)) == 0, r.stdout
