@@ -39,12 +39,32 @@ __global__ __launch_bounds__(256) void k_adj_faces(GridDev G, Range3 r, PerfectG
     if (range_cell(r, i, j, k)) adj::cell_faces<ND, LIM>(G, eos, A, i, j, k);
 }
 
-template <int ND, int RS, int DIR>
+// Cell c of the face range along d: the face between c - e_d and c touches an interior cell.
+__device__ __forceinline__ bool on_face_range(const GridDev& G, int d, int nd, const int* c)
+{
+    for (int e = 0; e < nd; ++e) {
+        bool const ok = (e == d) ? (c[e] >= G.ngh[e] && c[e] <= G.ngh[e] + G.n[e]) : adj::inside(G, e, c[e]);
+        if (!ok) return false;
+    }
+    return true;
+}
+
+// Every face direction in one pass: cell c handles the face on its low side along each dim
+// whose face range holds c.  Each face reads and rewrites its own two slots, so no two faces
+// touch the same memory.
+template <int ND, int RS>
 __global__ __launch_bounds__(256) void k_adj_flux(GridDev G, Range3 r, PerfectGas eos, AdjArrays A, double* __restrict__ partials)
 {
     int i, j, k;
     double lg = 0;
-    if (range_cell(r, i, j, k)) lg = adj::face_flux<ND, RS, DIR>(G, eos, A, i, j, k);
+    if (range_cell(r, i, j, k)) {
+        int const c[3] = {i, j, k};
+        if (on_face_range(G, 0, ND, c)) lg += adj::face_flux<ND, RS, 0>(G, eos, A, i, j, k);
+        if constexpr (ND >= 2)
+            if (on_face_range(G, 1, ND, c)) lg += adj::face_flux<ND, RS, 1>(G, eos, A, i, j, k);
+        if constexpr (ND == 3)
+            if (on_face_range(G, 2, ND, c)) lg += adj::face_flux<ND, RS, 2>(G, eos, A, i, j, k);
+    }
     put_partial(lg, partials);
 }
 
@@ -66,36 +86,46 @@ __global__ __launch_bounds__(256) void k_adj_gather(GridDev G, Range3 r, Perfect
     put_partial(lg, partials);
 }
 
-// dgamma[0] += sum of n partials, in a fixed order
+// dgamma[0] += sum of n partials, in a fixed order: thread t adds partials t, t + 1024, ...
+// in turn, then the workgroup tree.  The loads of 16 rounds are issued before their adds.
 __global__ __launch_bounds__(1024) void k_adj_fold(const double* __restrict__ partials, int64_t n, double* __restrict__ dgamma)
 {
+    constexpr int kU = 16;
     __shared__ double lds[16];
     double v = 0;
-    for (int64_t t = threadIdx.x; t < n; t += 1024) v += partials[t];
+    for (int64_t t0 = threadIdx.x; t0 < n; t0 += (int64_t)kU * 1024) {
+        double x[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            int64_t const t = t0 + (int64_t)u * 1024;
+            x[u] = t < n ? partials[t] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (t0 + (int64_t)u * 1024 < n) v += x[u];
+    }
     v = block_reduce<kSum>(v, lds, threadIdx.x, 1024);
     if (threadIdx.x == 0) dgamma[0] += v;
 }
 
 struct AdjRanges {
-    Range3 all, ring1, interior, flux[3];
+    Range3 ring2, ring1, interior, flux;  // flux: the union of the face ranges (interior + 1 cell on the high side of every dim)
 };
 
 AdjRanges adj_ranges(const GridDev& G, int nd)
 {
     AdjRanges R;
     for (int d = 0; d < 3; ++d) {
-        R.all.b[d] = 0;
-        R.all.e[d] = G.nwg[d];
         bool const act = d < nd;
+        R.ring2.b[d] = act ? G.ngh[d] - 2 : 0;
+        R.ring2.e[d] = act ? G.ngh[d] + G.n[d] + 2 : 1;
         R.ring1.b[d] = act ? G.ngh[d] - 1 : 0;
         R.ring1.e[d] = act ? G.ngh[d] + G.n[d] + 1 : 1;
         R.interior.b[d] = G.ngh[d];
         R.interior.e[d] = G.ngh[d] + G.n[d];
     }
-    for (int f = 0; f < 3; ++f) {
-        R.flux[f] = R.interior;
-        R.flux[f].e[f] += 1;
-    }
+    R.flux = R.interior;
+    for (int d = 0; d < nd; ++d) R.flux.e[d] += 1;
     return R;
 }
 
@@ -105,15 +135,15 @@ int64_t blocks_of(const Range3& r)
     return (int64_t)g.x * g.y * g.z;
 }
 
-template <int ND, int DIR>
+template <int ND>
 void launch_flux(int rs, const GridDev& G, const AdjRanges& R, const PerfectGas& eos, const AdjArrays& A, double* partials, int64_t& off,
                  hipStream_t s)
 {
-    Range3 const& r = R.flux[DIR];
+    Range3 const& r = R.flux;
     if (rs == kHLL)
-        hipLaunchKernelGGL((k_adj_flux<ND, kHLL, DIR>), range_grid(r), dim3(kBX, kBY), 0, s, G, r, eos, A, partials + off);
+        hipLaunchKernelGGL((k_adj_flux<ND, kHLL>), range_grid(r), dim3(kBX, kBY), 0, s, G, r, eos, A, partials + off);
     else
-        hipLaunchKernelGGL((k_adj_flux<ND, kHLLC, DIR>), range_grid(r), dim3(kBX, kBY), 0, s, G, r, eos, A, partials + off);
+        hipLaunchKernelGGL((k_adj_flux<ND, kHLLC>), range_grid(r), dim3(kBX, kBY), 0, s, G, r, eos, A, partials + off);
     off += blocks_of(r);
 }
 
@@ -122,9 +152,7 @@ void launch_flux(int rs, const GridDev& G, const AdjRanges& R, const PerfectGas&
 int64_t adjoint_partials(const GridDev& G, int nd)
 {
     AdjRanges const R = adj_ranges(G, nd);
-    int64_t n = blocks_of(R.ring1) + blocks_of(R.interior);
-    for (int d = 0; d < nd; ++d) n += blocks_of(R.flux[d]);
-    return n;
+    return blocks_of(R.ring1) + blocks_of(R.interior) + blocks_of(R.flux);
 }
 
 #define HCLPP_ADJ_LIM_SWITCH(lim, BODY)                      \
@@ -148,11 +176,9 @@ void launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const Perfec
     dim3 const blk(kBX, kBY);
     int64_t off = 0;
     HCLPP_ADJ_ND_SWITCH(nd, {
-        hipLaunchKernelGGL(k_adj_prim<ND>, range_grid(R.all), blk, 0, s, G, R.all, eos, A);
+        hipLaunchKernelGGL(k_adj_prim<ND>, range_grid(R.ring2), blk, 0, s, G, R.ring2, eos, A);
         HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_faces<ND, LIM>), range_grid(R.ring1), blk, 0, s, G, R.ring1, eos, A));
-        (launch_flux<ND, 0>)(rs, G, R, eos, A, partials, off, s);
-        if constexpr (ND >= 2) (launch_flux<ND, 1>)(rs, G, R, eos, A, partials, off, s);
-        if constexpr (ND == 3) (launch_flux<ND, 2>)(rs, G, R, eos, A, partials, off, s);
+        (launch_flux<ND>)(rs, G, R, eos, A, partials, off, s);
         HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_hancock<ND, LIM>), range_grid(R.ring1), blk, 0, s, G, R.ring1, eos, A,
                                                      partials + off));
         off += blocks_of(R.ring1);

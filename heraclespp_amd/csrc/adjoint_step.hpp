@@ -8,9 +8,9 @@
 // with dth = dt / 2 and F* the Riemann flux of (Uf[1][d] of the left cell, Uf[0][d] of the right one).
 // Given the adjoint lambda' of U' on the interior, the reverse step runs five passes; each
 // writes every output element exactly once (no atomics, results are deterministic):
-//   prim   (all cells)          p = prim(Ug)
+//   prim   (cells within 2 of the interior)   p = prim(Ug)
 //   faces  (ring-1 cells)       Uf, stored in the face-slot buffer
-//   flux   (faces touching the interior, one pass per dim)
+//   flux   (faces touching the interior, every dim in one pass)
 //                               lambda_F = dt/dV (lambda'(right) dS - lambda'(left) dS+), Riemann VJP,
 //                               the adjoints of the two face states overwrite their slots
 //   hancock (ring-1 cells)      adjoint of the Hancock predictor (face-state adjoints -> face-prim
