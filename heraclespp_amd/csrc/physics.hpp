@@ -12,9 +12,9 @@
 //    divisions, while the pressure recovery P = (gamma-1)(E - |m|^2 / 2 rho),
 //    which cancels at high Mach number, is evaluated in FP64.
 // Division is always written through Eos::div (or fdiv below) so the FP64
-// policy keeps the correctly-rounded quotient and the FP32 policy uses
-// v_rcp_f32.  Halving is a multiplication by 0.5, which is exact in IEEE
-// arithmetic.
+// policy keeps the correctly-rounded quotient (on normal-range operands, see
+// fdiv) and the FP32 policy uses v_rcp_f32.  Halving is a multiplication by
+// 0.5, which is exact in IEEE arithmetic.
 // Comments cite the reference file:line each function follows.
 #pragma once
 
