@@ -4,6 +4,8 @@
 // Five passes over 64 x 4 workgroups (x fastest, coalesced SoA planes); each output element
 // is written by exactly one thread, and the dL/dgamma contributions are reduced per
 // workgroup and folded in a fixed order, so the gradient is bit-reproducible run to run.
+// In a run of reverse steps, each step's gather pass also forms the primitives the next one
+// needs, so the primitive pass runs once per run.
 #include "adjoint_step.hpp"
 #include "reduce.hpp"
 
@@ -77,17 +79,45 @@ __global__ __launch_bounds__(256) void k_adj_hancock(GridDev G, Range3 r, Perfec
     put_partial(lg, partials);
 }
 
+// The state the next reverse step differentiates (the step before in time), whose primitives
+// the gather pass computes; null when there is none.
+struct NextState {
+    const double* rho;
+    const double* rhou;
+    const double* E;
+};
+
+__device__ __forceinline__ bool in_range(const Range3& r, int i, int j, int k)
+{
+    return i >= r.b[0] && i < r.e[0] && j >= r.b[1] && j < r.e[1] && k >= r.b[2] && k < r.e[2];
+}
+
+// Gather pass over `rg` (the cells within 2 of the interior): interior cells run the gather;
+// with `nx`, every cell also stores the next state's primitives for the next reverse step (the
+// gather reads no primitive, so the buffer is free).
 template <int ND, int LIM>
-__global__ __launch_bounds__(256) void k_adj_gather(GridDev G, Range3 r, PerfectGas eos, BcSpec bc, AdjArrays A, double* __restrict__ partials)
+__global__ __launch_bounds__(256) void k_adj_gather(GridDev G, Range3 rg, Range3 interior, PerfectGas eos, BcSpec bc, AdjArrays A, NextState nx,
+                                                    double* __restrict__ partials)
 {
     int i, j, k;
     double lg = 0;
-    if (range_cell(r, i, j, k)) lg = adj::cell_gather<ND, LIM>(G, eos, bc, A, i, j, k);
+    if (range_cell(rg, i, j, k)) {
+        if (in_range(interior, i, j, k)) lg = adj::cell_gather<ND, LIM>(G, eos, bc, A, i, j, k);
+        if (nx.rho) {
+            AdjArrays B = A;
+            B.rho = nx.rho;
+            B.rhou = nx.rhou;
+            B.E = nx.E;
+            adj::cell_prim<ND>(G, eos, B, i, j, k);
+        }
+    }
     put_partial(lg, partials);
 }
 
 // dgamma[0] += sum of n partials, in a fixed order: thread t adds partials t, t + 1024, ...
 // in turn, then the workgroup tree.  The loads of 16 rounds are issued before their adds.
+// (A separate launch: the kernel boundary publishes every workgroup's partial at once, where a
+// last-workgroup fold would need a device-scope fence, an L2 write-back on each XCD, per workgroup.)
 __global__ __launch_bounds__(1024) void k_adj_fold(const double* __restrict__ partials, int64_t n, double* __restrict__ dgamma)
 {
     constexpr int kU = 16;
@@ -152,7 +182,7 @@ void launch_flux(int rs, const GridDev& G, const AdjRanges& R, const PerfectGas&
 int64_t adjoint_partials(const GridDev& G, int nd)
 {
     AdjRanges const R = adj_ranges(G, nd);
-    return blocks_of(R.ring1) + blocks_of(R.interior) + blocks_of(R.flux);
+    return blocks_of(R.ring1) + blocks_of(R.ring2) + blocks_of(R.flux);
 }
 
 #define HCLPP_ADJ_LIM_SWITCH(lim, BODY)                      \
@@ -170,21 +200,27 @@ int64_t adjoint_partials(const GridDev& G, int nd)
     }
 
 void launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const AdjArrays& A,
-                         double* partials, double* dgamma, hipStream_t s)
+                         bool prims_ready, const double* next_rho, const double* next_rhou, const double* next_E, double* partials,
+                         double* dgamma, hipStream_t s)
 {
     AdjRanges const R = adj_ranges(G, nd);
     dim3 const blk(kBX, kBY);
+    NextState const nx{next_rho, next_rhou, next_E};
+    // The gather grid covers the cells within 2 of the interior whether or not it forms the next
+    // primitives, so its workgroups, and with them the dgamma partials, are the same either way:
+    // dgamma does not depend on how a run of reverse steps is split into calls.
+    Range3 const& rg = R.ring2;
     int64_t off = 0;
     HCLPP_ADJ_ND_SWITCH(nd, {
-        hipLaunchKernelGGL(k_adj_prim<ND>, range_grid(R.ring2), blk, 0, s, G, R.ring2, eos, A);
+        if (!prims_ready) hipLaunchKernelGGL(k_adj_prim<ND>, range_grid(R.ring2), blk, 0, s, G, R.ring2, eos, A);
         HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_faces<ND, LIM>), range_grid(R.ring1), blk, 0, s, G, R.ring1, eos, A));
         (launch_flux<ND>)(rs, G, R, eos, A, partials, off, s);
         HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_hancock<ND, LIM>), range_grid(R.ring1), blk, 0, s, G, R.ring1, eos, A,
                                                      partials + off));
         off += blocks_of(R.ring1);
-        HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_gather<ND, LIM>), range_grid(R.interior), blk, 0, s, G, R.interior, eos, bc, A,
+        HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_gather<ND, LIM>), range_grid(rg), blk, 0, s, G, rg, R.interior, eos, bc, A, nx,
                                                      partials + off));
-        off += blocks_of(R.interior);
+        off += blocks_of(rg);
     });
     hipLaunchKernelGGL(k_adj_fold, dim3(1), dim3(1024), 0, s, partials, off, dgamma);
 }

@@ -445,10 +445,14 @@ __host__ __device__ __forceinline__ double cell_gather(const GridDev& G, const P
 
 namespace hclpp {
 
-// adjoint.hip: number of workgroup partials one reverse step writes, and the launcher
-// (passes in the order above, then dgamma[0] += the fixed-order sum of the partials).
+// adjoint.hip: number of workgroup partials one reverse step writes at most, and the launcher
+// (passes in the order above, then dgamma[0] += the fixed-order sum of the partials).  With
+// prims_ready the primitive pass is skipped (a previous step's gather formed them); with
+// next_rho the gather pass forms the primitives of (next_rho, next_rhou, next_E) for the next
+// reverse step.
 int64_t adjoint_partials(const GridDev& G, int nd);
 void launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const adj::AdjArrays& A,
-                         double* partials, double* dgamma, hipStream_t s);
+                         bool prims_ready, const double* next_rho, const double* next_rhou, const double* next_E, double* partials,
+                         double* dgamma, hipStream_t s);
 
 }  // namespace hclpp
