@@ -28,6 +28,11 @@ calls.  Prints one JSON line:
              ratio to FP64, and the relative L1 distance of its density to the FP64
              density after the same number of steps (the FP64 path is bit-identical
              to the reference arithmetic)
+  rayleigh_taylor_512  the 3D Rayleigh-Taylor 512^3 run (Minmod + HLLC, uniform
+             gravity), 2 + 8 steps in FP64 and with FP32 storage: Gcell/s, kernel
+             time and fraction of the HBM peak at the algorithmic bytes, the
+             FP32/FP64 throughput ratio and the density L1 distance between the two
+             (with the default workload only; --no-3d skips it)
   adjoint    differentiable-Euler 1024^2 (VanAlbada + HLLC, 400 steps,
              checkpoint every 20): forward, checkpointed backward and reverse-
              kernel cost per step, and central-difference checks of the gradient
@@ -199,10 +204,10 @@ def calibrate_copy(nbytes: int):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
 
 
-def mixed_block(workload: str, steps: int, warmup: int, value64: float, rho64):
-    """The workload again with FP32 storage: throughput (same timing as the FP64 line) and the
-    density's relative L1 distance to the FP64 run after the same number of steps."""
-    import numpy as np
+def _timed_run(workload: str, precision: str, steps: int, warmup: int):
+    """W untimed then K timed fused steps of a workload: whole-step throughput (Gcell-updates/s,
+    wall clock between synchronisations), the fused kernel's mean time (HIP events) and the
+    interior density after all steps."""
     import torch
 
     from heraclespp_amd import _lib
@@ -213,7 +218,7 @@ def mixed_block(workload: str, steps: int, warmup: int, value64: float, rho64):
     grid = Grid(s.ndim, s.n, s.ng, s.geom, s.xmin, s.xmax)
     solver = HydroSolver(grid, PerfectGas(s.gamma, 1.0), reconstruction=s.reconstruction,
                          riemann_solver=s.riemann_solver, gravity=UniformGravity(s.gravity), bc=s.bc, cfl=s.cfl,
-                         precision="fp32")
+                         precision=precision)
     solver.init_from_prim(s.rho, s.u, s.P)
     del s.rho, s.u, s.P
     cells = 1
@@ -234,14 +239,46 @@ def mixed_block(workload: str, steps: int, warmup: int, value64: float, rho64):
     _lib.check(L.hclpp_profile_end(solver.ctx.handle, ctypes.byref(kms), ctypes.byref(nl)))
     clk = solver.finish()
     if clk.iter != warmup + steps:
-        raise RuntimeError(f"expected {warmup + steps} FP32 steps, clock counted {clk.iter}")
-    value = cells * steps / (t1 - t0) / 1e9
-    rho32 = solver.rho.cpu().numpy()[s.interior()]
-    l1 = float(np.sum(np.abs(rho32 - rho64)) / np.sum(np.abs(rho64)))
+        raise RuntimeError(f"expected {warmup + steps} {precision} steps, clock counted {clk.iter}")
+    rho = solver.rho.cpu().numpy()[s.interior()]
     del solver
-    return {"value": round(value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(kms.value / max(nl.value, 1), 5),
-            "ratio_to_fp64": round(value / value64, 3), "density_rel_l1_vs_fp64": float(f"{l1:.3e}"),
+    torch.cuda.empty_cache()
+    return cells * steps / (t1 - t0) / 1e9, kms.value / max(nl.value, 1), rho, cells, s.ndim
+
+
+def _rel_l1(a, b):
+    import numpy as np
+
+    return float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+
+
+def mixed_block(workload: str, steps: int, warmup: int, value64: float, rho64):
+    """The workload again with FP32 storage: throughput (same timing as the FP64 line) and the
+    density's relative L1 distance to the FP64 run after the same number of steps."""
+    value, kms, rho32, _, _ = _timed_run(workload, "fp32", steps, warmup)
+    return {"value": round(value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(kms, 5),
+            "ratio_to_fp64": round(value / value64, 3), "density_rel_l1_vs_fp64": float(f"{_rel_l1(rho32, rho64):.3e}"),
             "steps_compared": warmup + steps}
+
+
+def block_3d(steps: int, warmup: int):
+    """The 3D Rayleigh-Taylor 512^3 run (Minmod + HLLC, uniform gravity) in FP64 and with FP32
+    storage: throughput, kernel time and HBM fraction at the algorithmic bytes, the FP32/FP64
+    ratio and the density's relative L1 distance between the two after the same steps."""
+    out = {"workload": WORKLOADS["rayleigh_taylor"][2], "steps": steps, "warmup": warmup}
+    rho64 = None
+    for precision, word in (("fp64", 8), ("fp32", 4)):
+        value, kms, rho, cells, nd = _timed_run("rayleigh_taylor", precision, steps, warmup)
+        gbs = 2 * (nd + 2) * word * cells / (kms / 1e3) / 1e9
+        out[precision] = {"value": round(value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(kms, 4),
+                          "hbm_achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 5)}
+        if precision == "fp64":
+            rho64 = rho
+        else:
+            out["fp32"]["ratio_to_fp64"] = round(value / out["fp64"]["value"], 3)
+            out["fp32"]["density_rel_l1_vs_fp64"] = float(f"{_rel_l1(rho, rho64):.3e}")
+        del rho
+    return out
 
 
 def main():
@@ -257,6 +294,7 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-Euler adjoint measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-storage run of the same workload")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D Rayleigh-Taylor 512^3 FP64 / FP32 block")
     ap.add_argument("--adjoint-n", type=int, default=1024)
     ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--adjoint-fd-dirs", type=int, default=4)
@@ -361,6 +399,7 @@ def main():
         },
         "cpu_baseline": None,
         "mixed_fp32": None,
+        "rayleigh_taylor_512": None,
         "adjoint": None,
     }
     rho64 = solver.rho.cpu().numpy()[s.interior()]
@@ -369,6 +408,8 @@ def main():
     if a.precision == "fp64" and not a.no_mixed:
         line["mixed_fp32"] = mixed_block(a.workload, a.steps, a.warmup, value, rho64)
         torch.cuda.empty_cache()
+    if a.precision == "fp64" and a.workload == "kelvin_helmholtz" and not a.no_3d:
+        line["rayleigh_taylor_512"] = block_3d(8, 2)
     if not a.no_adjoint:
         line["adjoint"] = adjoint_block(a.adjoint_n, a.adjoint_steps, 20, a.adjoint_fd_dirs)
         torch.cuda.empty_cache()
