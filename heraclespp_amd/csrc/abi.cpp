@@ -731,10 +731,11 @@ namespace {
 
 // One reverse step (see hclpp_step_adjoint).  prims_ready: the primitives of (rho, rhou, E) are
 // already in the scratch buffer (the previous call formed them); next_*: the state of the next
-// reverse step, whose primitives this step's gather pass forms, or null.
+// reverse step, whose primitives this step's gather pass forms, or null.  *pending: in, the
+// dgamma partials the previous step left unfolded (0 if none); out, this step's (0 if fold_now).
 int step_adjoint(hclpp_ctx* c, double dt, const double* rho, const double* rhou, const double* E, const double* lrho_new,
                  const double* lrhou_new, const double* lE_new, double* lrho, double* lrhou, double* lE, double* dgamma, bool prims_ready,
-                 const double* next_rho, const double* next_rhou, const double* next_E)
+                 const double* next_rho, const double* next_rhou, const double* next_E, int64_t* pending, bool fold_now)
 {
     if (!c) return fail(HCLPP_ERR_ARG, "null context");
     if (c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step takes FP64 storage");
@@ -757,8 +758,8 @@ int step_adjoint(hclpp_ctx* c, double dt, const double* rho, const double* rhou,
         HCLPP_HIP(hipMalloc(&c->adj_partials, sizeof(double) * adjoint_partials(c->G, nd)));
     }
     adj::AdjArrays A{rho, rhou, E, c->adj_prim, c->adj_face, lrho_new, lrhou_new, lE_new, lrho, lrhou, lE, dt};
-    launch_step_adjoint(nd, c->cfg.riemann, c->cfg.limiter, c->G, c->eos, c->bc, A, prims_ready, next_rho, next_rhou, next_E,
-                        c->adj_partials, dgamma, c->stream);
+    *pending = launch_step_adjoint(nd, c->cfg.riemann, c->cfg.limiter, c->G, c->eos, c->bc, A, prims_ready, next_rho, next_rhou, next_E,
+                                   *pending, fold_now, c->adj_partials, dgamma, c->stream);
     HCLPP_LAUNCHED("reverse step");
     return HCLPP_OK;
 }
@@ -768,7 +769,9 @@ int step_adjoint(hclpp_ctx* c, double dt, const double* rho, const double* rhou,
 int hclpp_step_adjoint(hclpp_ctx* c, double dt, const double* rho, const double* rhou, const double* E, const double* lrho_new,
                        const double* lrhou_new, const double* lE_new, double* lrho, double* lrhou, double* lE, double* dgamma)
 {
-    return step_adjoint(c, dt, rho, rhou, E, lrho_new, lrhou_new, lE_new, lrho, lrhou, lE, dgamma, false, nullptr, nullptr, nullptr);
+    int64_t pending = 0;
+    return step_adjoint(c, dt, rho, rhou, E, lrho_new, lrhou_new, lE_new, lrho, lrhou, lE, dgamma, false, nullptr, nullptr, nullptr,
+                        &pending, true);
 }
 
 int hclpp_steps_dt(hclpp_ctx* c, int n, const double* dts, double* const* rho, double* const* rhou, double* const* E)
@@ -787,12 +790,17 @@ int hclpp_steps_adjoint(hclpp_ctx* c, int n, const double* dts, const double* co
 {
     if (!c) return fail(HCLPP_ERR_ARG, "null context");
     if (n < 0 || (n > 0 && (!dts || !rho || !rhou || !E || !lrho || !lrhou || !lE))) return fail(HCLPP_ERR_ARG, "null argument");
+    int64_t pending = 0;  // dgamma partials of the step just run, folded by the next one (or below)
     for (int k = n - 1; k >= 0; --k) {
         bool const more = k > 0;
-        if (more && (!rho[k - 1] || !rhou[k - 1] || !E[k - 1])) return fail(HCLPP_ERR_ARG, "null array");
-        int const rc = step_adjoint(c, dts[k], rho[k], rhou[k], E[k], lrho[k + 1], lrhou[k + 1], lE[k + 1], lrho[k], lrhou[k], lE[k], dgamma,
-                                    k < n - 1, more ? rho[k - 1] : nullptr, more ? rhou[k - 1] : nullptr, more ? E[k - 1] : nullptr);
-        if (rc) return rc;
+        int rc = (more && (!rho[k - 1] || !rhou[k - 1] || !E[k - 1])) ? fail(HCLPP_ERR_ARG, "null array") : HCLPP_OK;
+        if (!rc)
+            rc = step_adjoint(c, dts[k], rho[k], rhou[k], E[k], lrho[k + 1], lrhou[k + 1], lE[k + 1], lrho[k], lrhou[k], lE[k], dgamma,
+                              k < n - 1, more ? rho[k - 1] : nullptr, more ? rhou[k - 1] : nullptr, more ? E[k - 1] : nullptr, &pending, !more);
+        if (rc) {
+            if (pending > 0) launch_adjoint_fold(c->adj_partials, pending, dgamma, c->stream);  // the steps already run stay counted
+            return rc;
+        }
     }
     return HCLPP_OK;
 }

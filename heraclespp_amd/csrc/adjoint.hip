@@ -5,7 +5,8 @@
 // is written by exactly one thread, and the dL/dgamma contributions are reduced per
 // workgroup and folded in a fixed order, so the gradient is bit-reproducible run to run.
 // In a run of reverse steps, each step's gather pass also forms the primitives the next one
-// needs, so the primitive pass runs once per run.
+// needs, so the primitive pass runs once per run, and each step's faces pass folds the
+// previous step's dL/dgamma partials, so the fold runs on its own once per run.
 #include "adjoint_step.hpp"
 #include "reduce.hpp"
 
@@ -27,6 +28,30 @@ __device__ __forceinline__ void put_partial(double v, double* partials)
     if (tid == 0) partials[block_lin()] = v;
 }
 
+// dgamma[0] += the sum of n partials in a fixed order, by one workgroup of kT threads: thread t
+// adds partials t, t + kT, ... in turn (the loads of 16 rounds issued before their adds), then the
+// workgroup tree.  The same order wherever it runs, so dgamma does not depend on where.
+__device__ __forceinline__ void fold_partials(const double* __restrict__ partials, int64_t n, double* __restrict__ dgamma)
+{
+    constexpr int kU = 16;
+    __shared__ double lds[kT / 64];
+    int const tid = threadIdx.x + kBX * threadIdx.y;
+    double v = 0;
+    for (int64_t t0 = tid; t0 < n; t0 += (int64_t)kU * kT) {
+        double x[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            int64_t const t = t0 + (int64_t)u * kT;
+            x[u] = t < n ? partials[t] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (t0 + (int64_t)u * kT < n) v += x[u];
+    }
+    v = block_reduce<kSum>(v, lds, tid, kT);
+    if (tid == 0) dgamma[0] += v;
+}
+
 template <int ND>
 __global__ __launch_bounds__(256) void k_adj_prim(GridDev G, Range3 r, PerfectGas eos, AdjArrays A)
 {
@@ -34,11 +59,15 @@ __global__ __launch_bounds__(256) void k_adj_prim(GridDev G, Range3 r, PerfectGa
     if (range_cell(r, i, j, k)) adj::cell_prim<ND>(G, eos, A, i, j, k);
 }
 
+// Face states; the first workgroup also folds the `pending` partials the previous reverse step
+// left (the kernel boundary has published them, and this step writes its own only later).
 template <int ND, int LIM>
-__global__ __launch_bounds__(256) void k_adj_faces(GridDev G, Range3 r, PerfectGas eos, AdjArrays A)
+__global__ __launch_bounds__(256) void k_adj_faces(GridDev G, Range3 r, PerfectGas eos, AdjArrays A, const double* __restrict__ partials,
+                                                   int64_t pending, double* __restrict__ dgamma)
 {
     int i, j, k;
     if (range_cell(r, i, j, k)) adj::cell_faces<ND, LIM>(G, eos, A, i, j, k);
+    if (pending > 0 && block_lin() == 0) fold_partials(partials, pending, dgamma);
 }
 
 // Cell c of the face range along d: the face between c - e_d and c touches an interior cell.
@@ -114,28 +143,12 @@ __global__ __launch_bounds__(256) void k_adj_gather(GridDev G, Range3 rg, Range3
     put_partial(lg, partials);
 }
 
-// dgamma[0] += sum of n partials, in a fixed order: thread t adds partials t, t + 1024, ...
-// in turn, then the workgroup tree.  The loads of 16 rounds are issued before their adds.
-// (A separate launch: the kernel boundary publishes every workgroup's partial at once, where a
-// last-workgroup fold would need a device-scope fence, an L2 write-back on each XCD, per workgroup.)
-__global__ __launch_bounds__(1024) void k_adj_fold(const double* __restrict__ partials, int64_t n, double* __restrict__ dgamma)
+// The fold on its own, for the last step of a run.  (Folds run after a kernel boundary, which
+// publishes every workgroup's partial at once; a fold in the last workgroup of the gather pass
+// would need a device-scope fence, an L2 write-back on each XCD, in every workgroup.)
+__global__ __launch_bounds__(256) void k_adj_fold(const double* __restrict__ partials, int64_t n, double* __restrict__ dgamma)
 {
-    constexpr int kU = 16;
-    __shared__ double lds[16];
-    double v = 0;
-    for (int64_t t0 = threadIdx.x; t0 < n; t0 += (int64_t)kU * 1024) {
-        double x[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            int64_t const t = t0 + (int64_t)u * 1024;
-            x[u] = t < n ? partials[t] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u)
-            if (t0 + (int64_t)u * 1024 < n) v += x[u];
-    }
-    v = block_reduce<kSum>(v, lds, threadIdx.x, 1024);
-    if (threadIdx.x == 0) dgamma[0] += v;
+    fold_partials(partials, n, dgamma);
 }
 
 struct AdjRanges {
@@ -199,9 +212,9 @@ int64_t adjoint_partials(const GridDev& G, int nd)
     default: { constexpr int ND = 3; BODY; } break;    \
     }
 
-void launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const AdjArrays& A,
-                         bool prims_ready, const double* next_rho, const double* next_rhou, const double* next_E, double* partials,
-                         double* dgamma, hipStream_t s)
+int64_t launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const AdjArrays& A,
+                            bool prims_ready, const double* next_rho, const double* next_rhou, const double* next_E, int64_t pending,
+                            bool fold_now, double* partials, double* dgamma, hipStream_t s)
 {
     AdjRanges const R = adj_ranges(G, nd);
     dim3 const blk(kBX, kBY);
@@ -213,7 +226,8 @@ void launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const Perfec
     int64_t off = 0;
     HCLPP_ADJ_ND_SWITCH(nd, {
         if (!prims_ready) hipLaunchKernelGGL(k_adj_prim<ND>, range_grid(R.ring2), blk, 0, s, G, R.ring2, eos, A);
-        HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_faces<ND, LIM>), range_grid(R.ring1), blk, 0, s, G, R.ring1, eos, A));
+        HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_faces<ND, LIM>), range_grid(R.ring1), blk, 0, s, G, R.ring1, eos, A,
+                                                     partials, pending, dgamma));
         (launch_flux<ND>)(rs, G, R, eos, A, partials, off, s);
         HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_hancock<ND, LIM>), range_grid(R.ring1), blk, 0, s, G, R.ring1, eos, A,
                                                      partials + off));
@@ -222,7 +236,14 @@ void launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const Perfec
                                                      partials + off));
         off += blocks_of(rg);
     });
-    hipLaunchKernelGGL(k_adj_fold, dim3(1), dim3(1024), 0, s, partials, off, dgamma);
+    if (!fold_now) return off;
+    launch_adjoint_fold(partials, off, dgamma, s);
+    return 0;
+}
+
+void launch_adjoint_fold(const double* partials, int64_t n, double* dgamma, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_adj_fold, dim3(1), dim3(kBX, kBY), 0, s, partials, n, dgamma);
 }
 
 }  // namespace hclpp

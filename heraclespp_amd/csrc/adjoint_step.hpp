@@ -449,10 +449,14 @@ namespace hclpp {
 // (passes in the order above, then dgamma[0] += the fixed-order sum of the partials).  With
 // prims_ready the primitive pass is skipped (a previous step's gather formed them); with
 // next_rho the gather pass forms the primitives of (next_rho, next_rhou, next_E) for the next
-// reverse step.
+// reverse step.  pending: the number of partials the previous reverse step left unfolded in
+// `partials` (this step's faces pass adds them to dgamma first); fold_now: add this step's
+// partials before returning (returns 0), else return their number for the next step's `pending`.
+// launch_adjoint_fold adds n partials to dgamma on its own.
 int64_t adjoint_partials(const GridDev& G, int nd);
-void launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const adj::AdjArrays& A,
-                         bool prims_ready, const double* next_rho, const double* next_rhou, const double* next_E, double* partials,
-                         double* dgamma, hipStream_t s);
+int64_t launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const adj::AdjArrays& A,
+                            bool prims_ready, const double* next_rho, const double* next_rhou, const double* next_E, int64_t pending,
+                            bool fold_now, double* partials, double* dgamma, hipStream_t s);
+void launch_adjoint_fold(const double* partials, int64_t n, double* dgamma, hipStream_t s);
 
 }  // namespace hclpp
