@@ -225,6 +225,66 @@ def test_checkpoint_schedules_agree_bitwise(cuda):
         assert torch.equal(g[1], grads[0][1])
 
 
+def test_batched_reverse_steps_equal_single_steps(cuda):
+    """hclpp_steps_adjoint over 6 steps (after the first, each step's primitives come from the
+    gather pass of the step before, and its dgamma partials are folded by the faces pass of the
+    step after) equals 6 hclpp_step_adjoint calls bit for bit, dgamma included; a batch that stops
+    on a null state keeps the steps it ran, dgamma included, like a batch of just those steps."""
+    from heraclespp_amd.adjoint import StepEngine, _ptr_array
+
+    s, spec, rho0, u, P = _smooth(32, 6)
+    gamma = torch.tensor(1.4, dtype=torch.float64)
+    r, m, E = _cons0(rho0, u, P, gamma)
+    dts = cfl_schedule(spec, r, m, E, 1.4, 6, s.cfl)
+    n = len(dts)
+    eng = StepEngine(spec, 1.4)
+    states = [eng.load(r, m, E)] + [eng.new_state() for _ in range(n)]
+    eng.start(states[0])
+    eng.steps(states, dts)
+    rng = np.random.default_rng(5)
+    seed = eng.new_state()
+    for t in seed:
+        t.copy_(_t(rng.standard_normal(tuple(t.shape))))
+
+    def batched(lo, hi, broken=None):
+        """Reverse steps hi-1 .. lo in one call (state `broken`, if given, passed as null)."""
+        lams = [eng.new_state() for _ in range(lo, hi)] + [tuple(t.clone() for t in seed)]
+        dg = torch.zeros(1, dtype=torch.float64, device="cuda")
+        cols = []
+        for v in range(3):
+            arr = _ptr_array([U[v] for U in states[lo:hi]])
+            if broken is not None:
+                arr[broken - lo] = None
+            cols.append(arr)
+        lcols = [_ptr_array([L[v] for L in lams]) for v in range(3)]
+        rc = _lib.lib().hclpp_steps_adjoint(eng.ctx.handle, hi - lo, (ctypes.c_double * (hi - lo))(*dts[lo:hi]), *cols, *lcols,
+                                            _ptr(dg))
+        torch.cuda.synchronize()
+        return rc, lams, dg
+
+    rc, lams, dg = batched(0, n)
+    assert rc == _lib.HCLPP_OK
+    lam = tuple(t.clone() for t in seed)
+    dg1 = torch.zeros(1, dtype=torch.float64, device="cuda")
+    for k in range(n - 1, -1, -1):
+        out = eng.new_state()
+        eng.vjp(states[k], lam, out, dg1, dts[k])
+        lam = out
+    torch.cuda.synchronize()
+    for a, b in zip(lams[0], lam):
+        assert torch.equal(a, b)
+    assert torch.equal(dg, dg1)
+
+    # state 2 missing: steps 5, 4, 3 need it only as the next state of step 3, so steps 5 and 4 run
+    rc, lams_b, dg_b = batched(0, n, broken=2)
+    assert rc == _lib.HCLPP_ERR_ARG
+    rc, lams_p, dg_p = batched(4, n)
+    assert rc == _lib.HCLPP_OK
+    for a, b in zip(lams_b[4], lams_p[0]):
+        assert torch.equal(a, b)
+    assert torch.equal(dg_b, dg_p)
+
+
 def test_diff_euler_1024_gradient_64_directions(cuda):
     """The differentiable-Euler configuration at full size: 1024^2 periodic, VanAlbada + HLLC,
     400 steps with a checkpoint every 20; loss 1/2 sum (rho_T - rho_tgt)^2 dV with the target the
