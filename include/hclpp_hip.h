@@ -227,7 +227,8 @@ int hclpp_step_adjoint(hclpp_ctx* ctx, double dt, const double* rho, const doubl
  * state pointers given as arrays of n + 1 entries per variable (entries k and k + 1 must
  * differ; ping-pong buffers are fine).  hclpp_steps_adjoint runs the n reverse steps from
  * last to first: reverse step k uses state k (arrays of n entries) and maps adjoint k + 1
- * to adjoint k (arrays of n + 1 entries); every step adds to *dgamma. */
+ * to adjoint k (arrays of n + 1 entries); every step adds to *dgamma.  On an error return,
+ * the steps already run are complete, their adjoints written and their *dgamma terms added. */
 int hclpp_steps_dt(hclpp_ctx* ctx, int n, const double* dts, double* const* rho, double* const* rhou, double* const* E);
 int hclpp_steps_adjoint(hclpp_ctx* ctx, int n, const double* dts, const double* const* rho, const double* const* rhou,
                         const double* const* E, double* const* lrho, double* const* lrhou, double* const* lE, double* dgamma);
