@@ -147,12 +147,25 @@ __global__ __launch_bounds__(1024) void k_clock_advance(DevClock* c, const doubl
     double s[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) s[v] = 0;
-    for (int64_t t = threadIdx.x; t < nblocks; t += 1024) {
-        double const* p = partials + t * kPartialStride;
-        inv = reduce_combine<kMax>(inv, p[0]);
-        emin = reduce_combine<kMin>(emin, p[1]);
+    // rows t, t + 1024, ... of this thread, folded in that order; the loads of kR rows are issued
+    // before their folds, so a launch pays about one memory latency per kR rows, not per row
+    constexpr int kR = 4;
+    for (int64_t t0 = threadIdx.x; t0 < nblocks; t0 += kR * 1024) {
+        double row[kR][NV + 2];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) s[v] += p[2 + v];
+        for (int r = 0; r < kR; ++r) {
+            int64_t const t = t0 + r * 1024;
+#pragma unroll
+            for (int v = 0; v < NV + 2; ++v) row[r][v] = t < nblocks ? partials[t * kPartialStride + v] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            if (t0 + r * 1024 >= nblocks) break;
+            inv = reduce_combine<kMax>(inv, row[r][0]);
+            emin = reduce_combine<kMin>(emin, row[r][1]);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) s[v] += row[r][2 + v];
+        }
     }
     for (int64_t t = threadIdx.x; t < nghost; t += 1024) inv = reduce_combine<kMax>(inv, ghost_partials[t]);
     // all NV + 2 values reduced together: wave shuffles, one barrier, then thread 0 folds the
