@@ -5,6 +5,10 @@ Every translation unit is compiled with:
   * -ffp-contract=off: no a*b+c contraction anywhere, so the FP64 kernels
     round exactly like the reference's CPU build (bit-identical results);
   * -Xclang -target-feature -Xclang -packed-fp32-ops: no packed FP32 VALU ops.
+With HCLPP_IEEE_DIVISION=1 in the environment, every device division is the full
+IEEE sequence (-DHCLPP_IEEE_DIVISION; physics.hpp fdiv) and the FP64 kernels are
+about a fifth slower: results then equal IEEE division bit for bit also where values
+fall below about 1e-290.
 The fused-step sources are also scheduled for instruction-level parallelism
 (-amdgpu-sched-strategy=max-ilp): their kernels are bound by dependent FP64
 VALU chains, and on MI355X that schedule gives the FP64 march kernel 2 % (2D)
@@ -49,10 +53,15 @@ COMMON = [
 ]
 
 
+def _flags(src: str) -> list:
+    extra = ["-DHCLPP_IEEE_DIVISION"] if os.environ.get("HCLPP_IEEE_DIVISION") == "1" else []
+    return COMMON + extra + EXTRA.get(src, [])
+
+
 def _compile(src: str, verbose: bool) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + EXTRA.get(src, [])
+    cmd = [HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + _flags(src)
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -41,12 +41,16 @@ constexpr double kPi = 3.141592653589793238462643383279502884;
 // every quotient and returns what IEEE division gives for zero and infinite operands and NaNs.
 // Outside that range (finite non-zero operands with a numerator below 2^-969, a subnormal
 // divisor, reciprocal or quotient, or an exponent gap of 768 or more) the result may differ
-// from a / b.  Two instructions fewer per division, and the reciprocal part depends on b
-// alone, so the compiler computes it once for every division sharing a divisor.  The host
-// build divides.
+// from a / b, by at most an ulp or two.  In a hydro state that happens only where values
+// have decayed below about 1e-290 (the far field of a long blast-wave run), and such a
+// difference stays there: it never reaches a value of ordinary size.  Two instructions fewer
+// per division, and the reciprocal part depends on b alone, so the compiler computes it once
+// for every division sharing a divisor.  The host build divides, and so does a device build
+// with HCLPP_IEEE_DIVISION defined (heraclespp_amd/build.py: HCLPP_IEEE_DIVISION=1), for
+// results bit-identical to IEEE division everywhere, with FP64 kernels about a fifth slower.
 __host__ __device__ __forceinline__ double fdiv(double a, double b)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(HCLPP_IEEE_DIVISION)
     double y = __builtin_amdgcn_rcp(b);
     double e = __builtin_fma(-b, y, 1.0);
     y = __builtin_fma(y, e, y);
