@@ -205,9 +205,12 @@ def calibrate_copy(nbytes: int):
 
 
 def _timed_run(workload: str, precision: str, steps: int, warmup: int):
-    """W untimed then K timed fused steps of a workload: whole-step throughput (Gcell-updates/s,
-    wall clock between synchronisations), the fused kernel's mean time (HIP events) and the
-    interior density after all steps."""
+    """W untimed then K timed fused steps of a workload, the state resident in HBM.  Returns a
+    record: the setup (without its arrays), interior cells, wall seconds of the K steps (between
+    synchronisations), Gcell-updates/s, the fused kernel's mean time in ms (HIP events on its
+    stream), its launch plan, and the interior density after all steps."""
+    from types import SimpleNamespace
+
     import torch
 
     from heraclespp_amd import _lib
@@ -230,6 +233,7 @@ def _timed_run(workload: str, precision: str, steps: int, warmup: int):
     torch.cuda.synchronize()
     L = _lib.lib()
     _lib.check(L.hclpp_profile_begin(solver.ctx.handle, steps))
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         solver.step_async()
@@ -240,10 +244,14 @@ def _timed_run(workload: str, precision: str, steps: int, warmup: int):
     clk = solver.finish()
     if clk.iter != warmup + steps:
         raise RuntimeError(f"expected {warmup + steps} {precision} steps, clock counted {clk.iter}")
+    seglen, nwg, occ = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()
+    _lib.check(L.hclpp_fused_plan(solver.ctx.handle, ctypes.byref(seglen), ctypes.byref(nwg), ctypes.byref(occ)))
     rho = solver.rho.cpu().numpy()[s.interior()]
     del solver
     torch.cuda.empty_cache()
-    return cells * steps / (t1 - t0) / 1e9, kms.value / max(nl.value, 1), rho, cells, s.ndim
+    return SimpleNamespace(setup=s, cells=cells, wall=t1 - t0, value=cells * steps / (t1 - t0) / 1e9,
+                           kernel_ms=kms.value / max(nl.value, 1), rho=rho,
+                           plan={"segment_planes": seglen.value, "workgroups": nwg.value, "resident_per_cu": occ.value})
 
 
 def _rel_l1(a, b):
@@ -255,9 +263,9 @@ def _rel_l1(a, b):
 def mixed_block(workload: str, steps: int, warmup: int, value64: float, rho64):
     """The workload again with FP32 storage: throughput (same timing as the FP64 line) and the
     density's relative L1 distance to the FP64 run after the same number of steps."""
-    value, kms, rho32, _, _ = _timed_run(workload, "fp32", steps, warmup)
-    return {"value": round(value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(kms, 5),
-            "ratio_to_fp64": round(value / value64, 3), "density_rel_l1_vs_fp64": float(f"{_rel_l1(rho32, rho64):.3e}"),
+    r = _timed_run(workload, "fp32", steps, warmup)
+    return {"value": round(r.value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(r.kernel_ms, 5),
+            "ratio_to_fp64": round(r.value / value64, 3), "density_rel_l1_vs_fp64": float(f"{_rel_l1(r.rho, rho64):.3e}"),
             "steps_compared": warmup + steps}
 
 
@@ -268,16 +276,16 @@ def block_3d(steps: int, warmup: int):
     out = {"workload": WORKLOADS["rayleigh_taylor"][2], "steps": steps, "warmup": warmup}
     rho64 = None
     for precision, word in (("fp64", 8), ("fp32", 4)):
-        value, kms, rho, cells, nd = _timed_run("rayleigh_taylor", precision, steps, warmup)
-        gbs = 2 * (nd + 2) * word * cells / (kms / 1e3) / 1e9
-        out[precision] = {"value": round(value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(kms, 4),
+        r = _timed_run("rayleigh_taylor", precision, steps, warmup)
+        gbs = 2 * (r.setup.ndim + 2) * word * r.cells / (r.kernel_ms / 1e3) / 1e9
+        out[precision] = {"value": round(r.value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(r.kernel_ms, 4),
                           "hbm_achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 5)}
         if precision == "fp64":
-            rho64 = rho
+            rho64 = r.rho
         else:
-            out["fp32"]["ratio_to_fp64"] = round(value / out["fp64"]["value"], 3)
-            out["fp32"]["density_rel_l1_vs_fp64"] = float(f"{_rel_l1(rho, rho64):.3e}")
-        del rho
+            out["fp32"]["ratio_to_fp64"] = round(r.value / out["fp64"]["value"], 3)
+            out["fp32"]["density_rel_l1_vs_fp64"] = float(f"{_rel_l1(r.rho, rho64):.3e}")
+        del r
     return out
 
 
@@ -310,46 +318,12 @@ def main():
 
     import torch
 
-    from heraclespp_amd import _lib
-    from heraclespp_amd.hydro import Grid, PerfectGas, UniformGravity
-    from heraclespp_amd.solver import HydroSolver
-
     torch.cuda.set_device(0)
-    s = _setup(a.workload, WORKLOADS[a.workload][0])
-    grid = Grid(s.ndim, s.n, s.ng, s.geom, s.xmin, s.xmax)
-    solver = HydroSolver(grid, PerfectGas(s.gamma, 1.0), reconstruction=s.reconstruction,
-                         riemann_solver=s.riemann_solver, gravity=UniformGravity(s.gravity), bc=s.bc, cfl=s.cfl,
-                         precision=a.precision)
-    solver.init_from_prim(s.rho, s.u, s.P)
-    del s.rho, s.u, s.P
-    cells = 1
-    for d in range(s.ndim):
-        cells *= s.n[d]
-
-    solver.start_clock(1e300, a.warmup + a.steps + 1)
-    for _ in range(a.warmup):
-        solver.step_async()
-    torch.cuda.synchronize()
-    L = _lib.lib()
-    _lib.check(L.hclpp_profile_begin(solver.ctx.handle, a.steps))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        solver.step_async()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    kms = ctypes.c_double()
-    nl = ctypes.c_int()
-    _lib.check(L.hclpp_profile_end(solver.ctx.handle, ctypes.byref(kms), ctypes.byref(nl)))
-    clk = solver.finish()
-    if clk.iter != a.warmup + a.steps:
-        raise RuntimeError(f"expected {a.warmup + a.steps} steps, clock counted {clk.iter}")
-
-    seglen, nwg, occ = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()
-    _lib.check(L.hclpp_fused_plan(solver.ctx.handle, ctypes.byref(seglen), ctypes.byref(nwg), ctypes.byref(occ)))
-    wall = t1 - t0
-    value = cells * a.steps / wall / 1e9
-    kernel_s = kms.value / 1e3 / max(nl.value, 1)
+    r = _timed_run(a.workload, a.precision, a.steps, a.warmup)
+    s, cells = r.setup, r.cells
+    wall = r.wall
+    value = r.value
+    kernel_s = r.kernel_ms / 1e3
     word = 8 if a.precision == "fp64" else 4
     bytes_alg = 2 * (s.ndim + 2) * word * cells
     achieved = bytes_alg / kernel_s / 1e9
@@ -383,7 +357,7 @@ def main():
             "riemann_solver": s.riemann_solver,
             "storage": a.precision,
             "path": "fused (hclpp_step: march kernel + ghost fill + device dt)",
-            "launch": {"segment_planes": seglen.value, "workgroups": nwg.value, "resident_per_cu": occ.value},
+            "launch": r.plan,
         },
         "roofline": {
             "bound": "hbm",
@@ -402,9 +376,8 @@ def main():
         "rayleigh_taylor_512": None,
         "adjoint": None,
     }
-    rho64 = solver.rho.cpu().numpy()[s.interior()]
-    del solver
-    torch.cuda.empty_cache()
+    rho64 = r.rho
+    del r
     if a.precision == "fp64" and not a.no_mixed:
         line["mixed_fp32"] = mixed_block(a.workload, a.steps, a.warmup, value, rho64)
         torch.cuda.empty_cache()
