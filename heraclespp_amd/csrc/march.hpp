@@ -20,7 +20,8 @@
 //  * The Godunov update of a cell accumulates dims in the reference order
 //    (flux, geometric sources, gravity per dim, march dim last), so with the
 //    FP64 policy every conserved value is bit-identical to the reference's
-//    unfused sequence.
+//    unfused sequence (down to the range of physics.hpp fdiv: values below
+//    about 1e-290 may differ in their last bits).
 //  * Face states never touch HBM: per cell-update the kernel reads the
 //    conservative state once (plus halo re-reads served mostly by L2) and
 //    writes it once.  The epilogue reduces (in FP64) the CFL rate of the new
