@@ -1,38 +1,37 @@
 """Benchmark of the HERACLES++ hydro step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kelvin_helmholtz]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rayleigh_taylor]
 
 A step is one fused MUSCL-Hancock + Riemann + Godunov pass over the whole
 grid plus the ghost fill and the device-side dt reduction (hclpp_step).  The
-default workload is the 2D Kelvin-Helmholtz configuration (2048 x 2048,
-periodic, VanLeer + HLLC, gamma 1.4, FP64) on synthetic initial data.  The
-state is resident in HBM before the timed region; W untimed steps run
-first, then exactly K steps are timed between torch.cuda.synchronize()
-calls.  Prints one JSON line:
+default workload is the 3D Rayleigh-Taylor configuration (512^3, uniform
+gravity g_z = -0.1, periodic x/y, reflexive z walls, Minmod + HLLC, gamma
+1.4, FP64) on synthetic initial data.  The state is resident in HBM before
+the timed region; W untimed steps run first, then more untimed steps until at
+least --warmup-seconds of warm-up have passed (the GPU clocks settle), then
+exactly K steps are timed between torch.cuda.synchronize() calls.  Prints
+one JSON line:
   value      Gcell-updates/s = interior cells x K / wall seconds
   roofline   the fused kernel's algorithmic HBM bytes (read + write the
              conservative state once: 2 (ndim + 2) x 8 B per cell) divided by
              its mean duration, measured with HIP events on its stream,
-             against the 8 TB/s HBM3E peak; `traffic` is the measured
-             HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled
-             by a copy-kernel calibration) read from profiles/, or null
-             roofline.valu: the FP64 kernel is bound by its VALU issue rate, not
-             by HBM; from the committed counter summary (SQ_ACTIVE_INST_VALU per
-             cell) and the live kernel time, the fraction of the 1024 SIMDs'
-             issue slots used and the VALU roof in Gcell/s
+             against the 8 TB/s HBM3E peak (`frac`, also `hbm_frac`);
+             `traffic` is the measured HBM bytes per launch (rocprofv3
+             FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration)
+             read from profiles/, or null.  roofline.valu: from the committed
+             counter summary (SQ_ACTIVE_INST_VALU per cell) and the live kernel
+             time, the fraction of the 1024 SIMDs' FP64 issue slots used and the
+             VALU roof in Gcell/s; `bound` names the roof the kernel is closer to
   cpu_baseline  the CPU oracle (a restatement of the reference's unfused
-             kernels, OpenMP) on a bounded sample of the same workload, timed
-             on this host's cores
+             kernels, OpenMP) on a bounded sample of the same workload (a
+             smaller grid of the same configuration), timed on this host
   mixed_fp32 the same workload with FP32 storage (FP64 pressure recovery, dt and
              totals), timed the same way after the FP64 run: Gcell/s, kernel time,
              ratio to FP64, and the relative L1 distance of its density to the FP64
-             density after the same number of steps (the FP64 path is bit-identical
-             to the reference arithmetic)
-  rayleigh_taylor_512  the 3D Rayleigh-Taylor 512^3 run (Minmod + HLLC, uniform
-             gravity), 2 + 8 steps in FP64 and with FP32 storage: Gcell/s, kernel
-             time and fraction of the HBM peak at the algorithmic bytes, the
-             FP32/FP64 throughput ratio and the density L1 distance between the two
-             (with the default workload only; --no-3d skips it)
+             density after the configured step count (500 for Rayleigh-Taylor;
+             both runs continue untimed to it)
+  kelvin_helmholtz_2048  the 2D Kelvin-Helmholtz 2048^2 run (VanLeer + HLLC) in
+             FP64 and with FP32 storage, timed the same way (default workload only)
   adjoint    differentiable-Euler 1024^2 (VanAlbada + HLLC, 400 steps,
              checkpoint every 20): forward, checkpointed backward and reverse-
              kernel cost per step, and central-difference checks of the gradient
@@ -55,13 +54,15 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Gcell-updates/s & HBM roofline %, FP64 vs mixed FP32; adjoint/forward ratio"
 
 WORKLOADS = {
-    # name: (setup factory kwargs at full size, cpu-baseline sample kwargs, description)
-    "kelvin_helmholtz": ({"n": 2048}, {"n": 2048}, "2D Kelvin-Helmholtz 2048x2048 periodic, VanLeer+HLLC"),
-    "rayleigh_taylor": ({"n": 512}, {"n": 128}, "3D Rayleigh-Taylor 512^3, g_z=-0.1, Minmod+HLLC"),
+    # name: (setup factory kwargs at full size, cpu-baseline sample kwargs, description, steps after which the
+    # FP32-storage density is compared with the FP64 density: the configured run length, or a fixed window)
+    "kelvin_helmholtz": ({"n": 2048}, {"n": 2048}, "2D Kelvin-Helmholtz 2048x2048 periodic, VanLeer+HLLC", 2000),
+    "rayleigh_taylor": ({"n": 512}, {"n": 128},
+                        "3D Rayleigh-Taylor 512^3, g_z=-0.1, periodic x/y, reflexive z, Minmod+HLLC", 500),
     "sedov": ({"nr": 512, "nth": 128, "nph": 256}, {"nr": 128, "nth": 32, "nph": 64},
-              "3D spherical Sedov 512x128x256, HLL+VanLeer"),
-    "smooth_euler": ({"n": 1024}, {"n": 1024}, "2D smooth periodic Euler 1024x1024, VanAlbada+HLLC"),
-    "sod": ({"n": 1000}, {"n": 1000}, "1D Sod 1000 cells, Minmod+HLL"),
+              "3D spherical Sedov 512x128x256, HLL+VanLeer", 200),
+    "smooth_euler": ({"n": 1024}, {"n": 1024}, "2D smooth periodic Euler 1024x1024, VanAlbada+HLLC", 400),
+    "sod": ({"n": 1000}, {"n": 1000}, "1D Sod 1000 cells, Minmod+HLL", 500),
 }
 
 
@@ -116,10 +117,24 @@ def cpu_baseline(name: str, seconds: float):
         "value": cells * steps / el / 1e9,
         "unit": "Gcell-updates/s",
         "cores": threads,
+        "host_physical_cores": _physical_cores(),
         "kind": "port",
         "sample": f"{steps} steps of {name} at {'x'.join(str(v) for v in s.n[:s.ndim])} after 1 warm-up step "
-                  f"({el:.1f} s, OpenMP {threads} threads, unfused reference kernel order)",
+                  f"({el:.1f} s, OpenMP {threads} threads -- this job's CPU share -- unfused reference kernel "
+                  f"order; per-cell rate of the same configuration on a smaller grid)",
     }
+
+
+def _physical_cores():
+    """Physical cores of this host (distinct (core, socket) pairs of lscpu), or None."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu", "-p=core,socket"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    pairs = {line.strip() for line in out.splitlines() if line.strip() and not line.startswith("#")}
+    return len(pairs) or None
 
 
 def adjoint_block(n: int, steps: int, every: int, fd_dirs: int):
@@ -204,11 +219,13 @@ def calibrate_copy(nbytes: int):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
 
 
-def _timed_run(workload: str, precision: str, steps: int, warmup: int):
-    """W untimed then K timed fused steps of a workload, the state resident in HBM.  Returns a
-    record: the setup (without its arrays), interior cells, wall seconds of the K steps (between
-    synchronisations), Gcell-updates/s, the fused kernel's mean time in ms (HIP events on its
-    stream), its launch plan, and the interior density after all steps."""
+def _timed_run(workload: str, precision: str, steps: int, warmup: int, warmup_seconds: float, total: int):
+    """Fused steps of a workload with the state resident in HBM: `warmup` untimed steps, more until
+    `warmup_seconds` have passed (at most `total` - `steps` in all), then exactly `steps` timed steps
+    between synchronisations, then untimed steps up to `total` (if more).  Returns a record: the setup
+    (without its arrays), interior cells, wall seconds of the timed steps, Gcell-updates/s, the fused
+    kernel's mean time in ms over the timed steps (HIP events on its stream), its launch plan, the
+    warm-up step count, and the interior density after max(total, steps run) steps."""
     from types import SimpleNamespace
 
     import torch
@@ -227,10 +244,19 @@ def _timed_run(workload: str, precision: str, steps: int, warmup: int):
     cells = 1
     for d in range(s.ndim):
         cells *= s.n[d]
-    solver.start_clock(1e300, warmup + steps + 1)
+    open_ended = total <= 0  # no step count to reach: the run ends with the timed steps
+    cap = (1 << 30) if open_ended else max(total, warmup + steps)
+    solver.start_clock(1e300, cap)
     for _ in range(warmup):
         solver.step_async()
     torch.cuda.synchronize()
+    done = warmup
+    t_w = time.perf_counter()
+    while time.perf_counter() - t_w < warmup_seconds and (open_ended or done + steps < cap):
+        for _ in range(10 if open_ended else min(10, cap - steps - done)):
+            solver.step_async()
+            done += 1
+        torch.cuda.synchronize()
     L = _lib.lib()
     _lib.check(L.hclpp_profile_begin(solver.ctx.handle, steps))
     torch.cuda.synchronize()
@@ -241,16 +267,19 @@ def _timed_run(workload: str, precision: str, steps: int, warmup: int):
     t1 = time.perf_counter()
     kms, nl = ctypes.c_double(), ctypes.c_int()
     _lib.check(L.hclpp_profile_end(solver.ctx.handle, ctypes.byref(kms), ctypes.byref(nl)))
+    total = done + steps if open_ended else cap
+    for _ in range(total - done - steps):
+        solver.step_async()
     clk = solver.finish()
-    if clk.iter != warmup + steps:
-        raise RuntimeError(f"expected {warmup + steps} {precision} steps, clock counted {clk.iter}")
+    if clk.iter != total:
+        raise RuntimeError(f"expected {total} {precision} steps, clock counted {clk.iter}")
     seglen, nwg, occ = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()
     _lib.check(L.hclpp_fused_plan(solver.ctx.handle, ctypes.byref(seglen), ctypes.byref(nwg), ctypes.byref(occ)))
     rho = solver.rho.cpu().numpy()[s.interior()]
     del solver
     torch.cuda.empty_cache()
     return SimpleNamespace(setup=s, cells=cells, wall=t1 - t0, value=cells * steps / (t1 - t0) / 1e9,
-                           kernel_ms=kms.value / max(nl.value, 1), rho=rho,
+                           kernel_ms=kms.value / max(nl.value, 1), rho=rho, warmup_steps=done, total=total,
                            plan={"segment_planes": seglen.value, "workgroups": nwg.value, "resident_per_cu": occ.value})
 
 
@@ -260,41 +289,47 @@ def _rel_l1(a, b):
     return float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
 
 
-def mixed_block(workload: str, steps: int, warmup: int, value64: float, rho64):
+def _hbm(r, precision):
+    word = 8 if precision == "fp64" else 4
+    gbs = 2 * (r.setup.ndim + 2) * word * r.cells / (r.kernel_ms / 1e3) / 1e9
+    return round(gbs, 1), round(gbs / HBM_PEAK_GBS, 5)
+
+
+def mixed_block(workload: str, a, value64: float, rho64, total: int):
     """The workload again with FP32 storage: throughput (same timing as the FP64 line) and the
-    density's relative L1 distance to the FP64 run after the same number of steps."""
-    r = _timed_run(workload, "fp32", steps, warmup)
+    density's relative L1 distance to the FP64 run after the same `total` steps."""
+    r = _timed_run(workload, "fp32", a.steps, a.warmup, a.warmup_seconds, total)
+    gbs, frac = _hbm(r, "fp32")
     return {"value": round(r.value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(r.kernel_ms, 5),
-            "ratio_to_fp64": round(r.value / value64, 3), "density_rel_l1_vs_fp64": float(f"{_rel_l1(r.rho, rho64):.3e}"),
-            "steps_compared": warmup + steps}
+            "hbm_achieved_gbs": gbs, "hbm_frac": frac, "ratio_to_fp64": round(r.value / value64, 3),
+            "density_rel_l1_vs_fp64": float(f"{_rel_l1(r.rho, rho64):.3e}"), "steps_compared": r.total,
+            "warmup_steps_run": r.warmup_steps}
 
 
-def block_3d(steps: int, warmup: int):
-    """The 3D Rayleigh-Taylor 512^3 run (Minmod + HLLC, uniform gravity) in FP64 and with FP32
-    storage: throughput, kernel time and HBM fraction at the algorithmic bytes, the FP32/FP64
-    ratio and the density's relative L1 distance between the two after the same steps."""
-    out = {"workload": WORKLOADS["rayleigh_taylor"][2], "steps": steps, "warmup": warmup}
-    rho64 = None
-    for precision, word in (("fp64", 8), ("fp32", 4)):
-        r = _timed_run("rayleigh_taylor", precision, steps, warmup)
-        gbs = 2 * (r.setup.ndim + 2) * word * r.cells / (r.kernel_ms / 1e3) / 1e9
-        out[precision] = {"value": round(r.value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(r.kernel_ms, 4),
-                          "hbm_achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 5)}
-        if precision == "fp64":
-            rho64 = r.rho
-        else:
-            out["fp32"]["ratio_to_fp64"] = round(r.value / out["fp64"]["value"], 3)
-            out["fp32"]["density_rel_l1_vs_fp64"] = float(f"{_rel_l1(r.rho, rho64):.3e}")
-        del r
+def kh_block(a):
+    """The 2D Kelvin-Helmholtz 2048^2 run (VanLeer + HLLC) in FP64 and with FP32 storage: throughput,
+    kernel time and HBM fraction at the algorithmic bytes, and the FP32/FP64 density L1 after its
+    configured 2000 steps."""
+    out = {"workload": WORKLOADS["kelvin_helmholtz"][2], "steps": a.steps}
+    total = WORKLOADS["kelvin_helmholtz"][3]
+    r = _timed_run("kelvin_helmholtz", "fp64", a.steps, a.warmup, a.warmup_seconds, total)
+    gbs, frac = _hbm(r, "fp64")
+    out["fp64"] = {"value": round(r.value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(r.kernel_ms, 5),
+                   "hbm_achieved_gbs": gbs, "hbm_frac": frac, "warmup_steps_run": r.warmup_steps}
+    rho64, v64, total = r.rho, r.value, r.total
+    del r
+    out["fp32"] = mixed_block("kelvin_helmholtz", a, v64, rho64, total)
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="kelvin_helmholtz", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup-seconds", type=float, default=1.0,
+                    help="keep stepping untimed until this much warm-up time has passed")
+    ap.add_argument("--workload", default="rayleigh_taylor", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
                     help="fp32: FP32 storage and arithmetic with FP64 pressure recovery, dt and totals")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -302,7 +337,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-Euler adjoint measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-storage run of the same workload")
-    ap.add_argument("--no-3d", action="store_true", help="skip the 3D Rayleigh-Taylor 512^3 FP64 / FP32 block")
+    ap.add_argument("--no-kh", action="store_true", help="skip the 2D Kelvin-Helmholtz 2048^2 FP64 / FP32 block")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="stop after the timed steps instead of continuing to the configured step count")
     ap.add_argument("--adjoint-n", type=int, default=1024)
     ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--adjoint-fd-dirs", type=int, default=4)
@@ -319,7 +356,8 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
-    r = _timed_run(a.workload, a.precision, a.steps, a.warmup)
+    total = 0 if (a.no_accuracy or a.no_mixed or a.precision != "fp64") else WORKLOADS[a.workload][3]
+    r = _timed_run(a.workload, a.precision, a.steps, a.warmup, a.warmup_seconds, total)
     s, cells = r.setup, r.cells
     wall = r.wall
     value = r.value
@@ -327,15 +365,22 @@ def main():
     word = 8 if a.precision == "fp64" else 4
     bytes_alg = 2 * (s.ndim + 2) * word * cells
     achieved = bytes_alg / kernel_s / 1e9
+    hbm_frac = achieved / HBM_PEAK_GBS
     traffic = _traffic(a.workload, a.precision)
     units = _valu_units(a.workload, a.precision)
-    # FP64 VALU issue capacity: 1024 SIMDs, one wave64 instruction per 4 cycles each, at 2.4 GHz
+    # FP64 VALU issue capacity: 1024 SIMDs, one wave64 FP64 instruction per 4 cycles each, at 2.4 GHz
     valu = None
+    bound = "hbm"
     if units is not None:
         issued = units * cells / 64
         capacity = 1024 * kernel_s * 2.4e9 / 4
+        roof = 39.3e12 / units / 1e9
         valu = {"issue_units_per_cell": units, "busy_frac": round(issued / capacity, 4),
-                "roof_gcells": round(39.3e12 / units / 1e9, 3)}
+                "roof_gcells": round(roof, 3), "kernel_gcells": round(cells / kernel_s / 1e9, 4),
+                "frac_of_roof": round(cells / kernel_s / 1e9 / roof, 4),
+                "source": f"profiles/k_step_march_{a.workload}_{a.precision}_valu.txt"}
+        if issued / capacity > hbm_frac:
+            bound = "valu"
     line = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -358,13 +403,16 @@ def main():
             "storage": a.precision,
             "path": "fused (hclpp_step: march kernel + ghost fill + device dt)",
             "launch": r.plan,
+            "warmup_steps_run": r.warmup_steps,
+            "warmup_seconds_min": a.warmup_seconds,
         },
         "roofline": {
-            "bound": "hbm",
+            "bound": bound,
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "frac": round(hbm_frac, 5),
+            "hbm_frac": round(hbm_frac, 5),
             "traffic": traffic,
             "kernel": "k_step_march",
             "kernel_ms": round(kernel_s * 1e3, 5),
@@ -373,16 +421,16 @@ def main():
         },
         "cpu_baseline": None,
         "mixed_fp32": None,
-        "rayleigh_taylor_512": None,
+        "kelvin_helmholtz_2048": None,
         "adjoint": None,
     }
-    rho64 = r.rho
+    rho64, total = r.rho, r.total
     del r
     if a.precision == "fp64" and not a.no_mixed:
-        line["mixed_fp32"] = mixed_block(a.workload, a.steps, a.warmup, value, rho64)
+        line["mixed_fp32"] = mixed_block(a.workload, a, value, rho64, total)
         torch.cuda.empty_cache()
-    if a.precision == "fp64" and a.workload == "kelvin_helmholtz" and not a.no_3d:
-        line["rayleigh_taylor_512"] = block_3d(8, 2)
+    if a.precision == "fp64" and a.workload == "rayleigh_taylor" and not a.no_kh:
+        line["kelvin_helmholtz_2048"] = kh_block(a)
     if not a.no_adjoint:
         line["adjoint"] = adjoint_block(a.adjoint_n, a.adjoint_steps, 20, a.adjoint_fd_dirs)
         torch.cuda.empty_cache()

@@ -2,6 +2,8 @@
 
 The library must have been built in-tree (``python -m heraclespp_amd.build``);
 there is no fallback of any kind: a missing or unloadable library raises.
+HCLPP_LIB in the environment names another build of the same C-ABI to load
+(libhclpp_hip_ieee.so: IEEE division in every kernel).
 """
 
 from __future__ import annotations
@@ -10,7 +12,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhclpp_hip.so")
+LIB_PATH = os.environ.get("HCLPP_LIB") or os.path.join(HERE, "libhclpp_hip.so")
 
 HCLPP_OK = 0
 HCLPP_ERR_ARG = 1
@@ -117,11 +119,11 @@ def lib() -> ctypes.CDLL:
         "hclpp_min_internal_energy": [vp, rp, dp, dp, dp, ctypes.POINTER(d)],
         "hclpp_integrate": [vp, rp, dp, ctypes.POINTER(d)],
         "hclpp_fill_ghosts": [vp, dp, dp, dp, dp],
-        "hclpp_clock_start": [vp, d, d, d, ip, dp, dp, dp],
+        "hclpp_clock_start": [vp, d, ip, d, d, ip, dp, dp, dp],
         "hclpp_clock_read": [vp, ctypes.POINTER(Clock)],
         "hclpp_step": [vp, dp, dp, dp, dp, dp, dp],
         "hclpp_step_fx": [vp, dp, dp, dp, dp, dp, dp, dp, dp],
-        "hclpp_clock_start_f32": [vp, d, d, d, ip, dp, dp, dp],
+        "hclpp_clock_start_f32": [vp, d, ip, d, d, ip, dp, dp, dp],
         "hclpp_step_f32": [vp, dp, dp, dp, dp, dp, dp],
         "hclpp_step_fx_f32": [vp, dp, dp, dp, dp, dp, dp, dp, dp],
         "hclpp_fused_plan": [vp, ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ip)],

@@ -88,7 +88,7 @@ class StepEngine:
         return U
 
     def start(self, U, cfl: float = 1.0, max_iter: int = 2 ** 31 - 1):
-        check(lib().hclpp_clock_start(self.ctx.handle, 0.0, 1e300, cfl, max_iter, _ptr(U[0]), _ptr(U[1]), _ptr(U[2])))
+        check(lib().hclpp_clock_start(self.ctx.handle, 0.0, 0, 1e300, cfl, max_iter, _ptr(U[0]), _ptr(U[1]), _ptr(U[2])))
 
     def step(self, src, dst, dt: float):
         check(lib().hclpp_step_dt(self.ctx.handle, float(dt), _ptr(src[0]), _ptr(src[1]), _ptr(src[2]), _ptr(dst[0]),

@@ -5,10 +5,12 @@ Every translation unit is compiled with:
   * -ffp-contract=off: no a*b+c contraction anywhere, so the FP64 kernels
     round exactly like the reference's CPU build (bit-identical results);
   * -Xclang -target-feature -Xclang -packed-fp32-ops: no packed FP32 VALU ops.
-With HCLPP_IEEE_DIVISION=1 in the environment, every device division is the full
-IEEE sequence (-DHCLPP_IEEE_DIVISION; physics.hpp fdiv) and the FP64 kernels are
-about a fifth slower: results then equal IEEE division bit for bit also where values
-fall below about 1e-290.
+The same sources built with -DHCLPP_IEEE_DIVISION give libhclpp_hip_ieee.so
+(`--ieee`, or HCLPP_IEEE_DIVISION=1 in the environment for the default name): every
+device division is then the full IEEE sequence (physics.hpp fdiv) and the FP64
+kernels are about a fifth slower, with results equal to IEEE division bit for bit
+also where values fall below about 1e-290.  The package loads the library named by
+HCLPP_LIB when that is set (heraclespp_amd/_lib.py).
 The fused-step sources are also scheduled for instruction-level parallelism
 (-amdgpu-sched-strategy=max-ilp): their kernels are bound by dependent FP64
 VALU chains, and on MI355X that schedule gives the FP64 march kernel 2 % (2D)
@@ -32,6 +34,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhclpp_hip.so")
+LIB_IEEE = os.path.join(HERE, "libhclpp_hip_ieee.so")
 
 SOURCES = ["ref_kernels.hip", "step_fused.hip", "step_fused_fx.hip", "adjoint.hip", "abi.cpp"]
 EXTRA = {src: ["-mllvm", "-amdgpu-sched-strategy=max-ilp"] for src in ("step_fused.hip", "step_fused_fx.hip")}
@@ -53,15 +56,16 @@ COMMON = [
 ]
 
 
-def _flags(src: str) -> list:
-    extra = ["-DHCLPP_IEEE_DIVISION"] if os.environ.get("HCLPP_IEEE_DIVISION") == "1" else []
+def _flags(src: str, ieee: bool) -> list:
+    extra = ["-DHCLPP_IEEE_DIVISION"] if ieee else []
     return COMMON + extra + EXTRA.get(src, [])
 
 
-def _compile(src: str, verbose: bool) -> str:
-    os.makedirs(OBJDIR, exist_ok=True)
-    obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + _flags(src)
+def _compile(src: str, verbose: bool, ieee: bool) -> str:
+    objdir = os.path.join(OBJDIR, "ieee") if ieee else OBJDIR
+    os.makedirs(objdir, exist_ok=True)
+    obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + _flags(src, ieee)
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -73,27 +77,45 @@ def _compile(src: str, verbose: bool) -> str:
     return obj
 
 
-def build(jobs: int | None = None, verbose: bool = False) -> str:
+def build(jobs: int | None = None, verbose: bool = False, ieee: bool | None = None) -> str:
+    """Compile and link one library: the default build, or with ieee=True libhclpp_hip_ieee.so
+    (ieee=None: the default build, with IEEE division when HCLPP_IEEE_DIVISION=1)."""
+    env_ieee = os.environ.get("HCLPP_IEEE_DIVISION") == "1"
+    lib = LIB_IEEE if ieee else LIB
+    ieee = bool(ieee) or env_ieee
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 1)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda s: _compile(s, verbose, ieee), SOURCES))
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
+
+
+def build_all(jobs: int | None = None, verbose: bool = False) -> list:
+    """Both libraries, compiled concurrently."""
+    jobs = jobs or (os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=2) as ex:
+        futs = [ex.submit(build, max(1, jobs // 2), verbose, flag) for flag in (None, True)]
+        return [f.result() for f in futs]
 
 
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=None)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--ieee", action="store_true", help="build libhclpp_hip_ieee.so (IEEE division everywhere)")
+    ap.add_argument("--all", action="store_true", help="build both libraries")
     a = ap.parse_args()
-    print(build(a.j, a.verbose))
+    if a.all:
+        print("\n".join(build_all(a.j, a.verbose)))
+    else:
+        print(build(a.j, a.verbose, True if a.ieee else None))
 
 
 if __name__ == "__main__":

@@ -596,14 +596,16 @@ static int fused_step(hclpp_ctx* c, const S* rho, const S* rhou, const S* E, con
 }
 }  // extern "C++"
 
-static int clock_reset(hclpp_ctx* c, double t0, double t_end, double cfl, int max_iter)
+static int clock_reset(hclpp_ctx* c, double t0, int iter0, double t_end, double cfl, int max_iter)
 {
     if (!(cfl > 0) || max_iter < 1) return fail(HCLPP_ERR_ARG, "cfl must be > 0 and max_iter >= 1");
+    if (iter0 < 0) return fail(HCLPP_ERR_ARG, "iter0 must be >= 0");
     if (c->cfg.ng < 2) return fail(HCLPP_ERR_ARG, "the fused step needs ng >= 2 (two-cell reconstruction stencil)");
     DevClock* h = c->host_clock;
     HCLPP_HIP(hipStreamSynchronize(c->stream));  // the pinned staging buffer is reused
     std::memset(h, 0, sizeof(DevClock));
     h->pub.t = t0;
+    h->pub.iter = iter0;
     h->pub.t_end = t_end;
     h->pub.cfl = cfl;
     h->pub.max_iter = max_iter;
@@ -617,11 +619,12 @@ static int clock_reset(hclpp_ctx* c, double t0, double t_end, double cfl, int ma
     return ensure_partials(c, reduce_blocks(all));
 }
 
-int hclpp_clock_start_f32(hclpp_ctx* c, double t0, double t_end, double cfl, int max_iter, const float* rho, const float* rhou, const float* E)
+int hclpp_clock_start_f32(hclpp_ctx* c, double t0, int iter0, double t_end, double cfl, int max_iter, const float* rho,
+                          const float* rhou, const float* E)
 {
     if (!c) return fail(HCLPP_ERR_ARG, "null context");
     if (c->cfg.precision != HCLPP_FP32_STORAGE) return fail(HCLPP_ERR_ARG, "the context was not created for FP32 storage");
-    int rc = clock_reset(c, t0, t_end, cfl, max_iter);
+    int rc = clock_reset(c, t0, iter0, t_end, cfl, max_iter);
     if (rc) return rc;
     launch_state_rate_f32(c->cfg.ndim, c->cfg.geom, c->G, c->eos32, rho, rhou, E, c->partials, c->scalars, c->stream);
     launch_clock_start(c->clock, c->scalars, c->stream);
@@ -657,28 +660,18 @@ int hclpp_cast_f32_to_f64(hclpp_ctx* c, const float* in, double* out, long long 
     return HCLPP_OK;
 }
 
-int hclpp_clock_start(hclpp_ctx* c, double t0, double t_end, double cfl, int max_iter, const double* rho, const double* rhou, const double* E)
+int hclpp_clock_start(hclpp_ctx* c, double t0, int iter0, double t_end, double cfl, int max_iter, const double* rho, const double* rhou,
+                      const double* E)
 {
     if (!c) return fail(HCLPP_ERR_ARG, "null context");
     if (c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_ARG, "the context was created for FP32 storage");
-    if (!(cfl > 0) || max_iter < 1) return fail(HCLPP_ERR_ARG, "cfl must be > 0 and max_iter >= 1");
-    if (c->cfg.ng < 2) return fail(HCLPP_ERR_ARG, "the fused step needs ng >= 2 (two-cell reconstruction stencil)");
-    DevClock* h = c->host_clock;
-    HCLPP_HIP(hipStreamSynchronize(c->stream));  // the pinned staging buffer is reused
-    std::memset(h, 0, sizeof(DevClock));
-    h->pub.t = t0;
-    h->pub.t_end = t_end;
-    h->pub.cfl = cfl;
-    h->pub.max_iter = max_iter;
-    h->pub.min_eint = NAN;
-    HCLPP_HIP(hipMemcpyAsync(c->clock, h, sizeof(DevClock), hipMemcpyHostToDevice, c->stream));
+    int rc = clock_reset(c, t0, iter0, t_end, cfl, max_iter);
+    if (rc) return rc;
     Range3 all;
     for (int d = 0; d < 3; ++d) {
         all.b[d] = 0;
         all.e[d] = c->G.nwg[d];
     }
-    int rc = ensure_partials(c, reduce_blocks(all));
-    if (rc) return rc;
     // main.cpp:355: dt from the ghost-inclusive state
     launch_time_step_cons(c->cfg.ndim, c->cfg.geom, c->G, all, c->eos, rho, rhou, E, c->partials, c->scalars, c->stream);
     launch_clock_start(c->clock, c->scalars, c->stream);

@@ -39,11 +39,20 @@ constexpr double kPi = 3.141592653589793238462643383279502884;
 //     exponent(a) - exponent(b) < 768,
 // so there the result is bit-identical to a / b.  v_div_fixup_f64 stays: it sets the sign of
 // every quotient and returns what IEEE division gives for zero and infinite operands and NaNs.
-// Outside that range (finite non-zero operands with a numerator below 2^-969, a subnormal
-// divisor, reciprocal or quotient, or an exponent gap of 768 or more) the result may differ
-// from a / b, by at most an ulp or two.  In a hydro state that happens only where values
-// have decayed below about 1e-290 (the far field of a long blast-wave run), and such a
-// difference stays there: it never reaches a value of ordinary size.  Two instructions fewer
+// Outside that range:
+//   * finite non-zero operands with a numerator below 2^-969, a subnormal quotient or an
+//     exponent gap of 768 or more: the result may differ from a / b by an ulp;
+//   * a subnormal divisor: v_rcp_f64 overflows to infinity and the result is NaN, also where
+//     a / b is finite;
+//   * a divisor above 2^1022: the reciprocal is subnormal and the result may be 0 where a / b
+//     is not.
+// The step divides with fdiv only by densities, density x kb and the EoS constants, cell
+// sizes and volumes, wave-speed differences and the HLLC star-state denominators (which are
+// of the order of the sound speed times the density).  The slope limiters divide by
+// differences of neighbouring values, which do reach the subnormal range where a blast
+// wave's far field has decayed, and by squares of their quotient, which overflow: they use
+// the IEEE quotient (Eos::ratio).  Where a numerator has decayed below about 1e-290 the
+// one-ulp differences stay there: they never reach a value of ordinary size.  Two instructions fewer
 // per division, and the reciprocal part depends on b alone, so the compiler computes it once
 // for every division sharing a divisor.  The host build divides, and so does a device build
 // with HCLPP_IEEE_DIVISION defined (heraclespp_amd/build.py: HCLPP_IEEE_DIVISION=1), for
@@ -94,6 +103,8 @@ struct PerfectGas {
     double mmw;
 
     __host__ __device__ static __forceinline__ double div(double a, double b) { return fdiv(a, b); }
+    // IEEE division for the slope limiters, whose operands range over every magnitude (fdiv above)
+    __host__ __device__ static __forceinline__ double ratio(double a, double b) { return a / b; }
     // perfect_gas.hpp:69-72
     __host__ __device__ __forceinline__ double temp_from_pres(double rho, double P) const { return fdiv(P * mmw * kMp, rho * kKb); }
     // perfect_gas.hpp:52-56
@@ -152,6 +163,7 @@ struct PerfectGasMixed {
         return a / b;
 #endif
     }
+    __host__ __device__ static __forceinline__ float ratio(float a, float b) { return div(a, b); }
     __host__ __device__ __forceinline__ float evol_from_pres(float, float P) const { return P * inv_gamma_m1; }
     __host__ __device__ __forceinline__ float sound(float rho, float P) const { return sqrtf(gamma * div(P, rho)); }
     // E - |m|^2 / (2 rho) and (gamma - 1) times it, in FP64
@@ -283,7 +295,8 @@ __host__ __device__ __forceinline__ State<ND, typename Eos::real> flux_of_cons(c
 }
 
 // ---------------------------------------------------------------- limiters
-// slope_limiters.hpp:18-87.  diffR = (v[i+1]-v[i])/dx, diffL = (v[i]-v[i-1])/dx
+// slope_limiters.hpp:18-87.  diffR = (v[i+1]-v[i])/dx, diffL = (v[i]-v[i-1])/dx.
+// Every division is Eos::ratio: dL can be subnormal and (r + 1)^2 can overflow.
 template <int LIM, class Eos, class T>
 __host__ __device__ __forceinline__ T limited_slope_t(T dR, T dL)
 {
@@ -291,14 +304,14 @@ __host__ __device__ __forceinline__ T limited_slope_t(T dR, T dL)
         return 0;
     } else {
         if (dL * dR > 0) {
-            T const r = Eos::div(dR, dL);
+            T const r = Eos::ratio(dR, dL);
             if constexpr (LIM == kVanLeer) {
-                return Eos::div(T(0.5) * (dR + dL) * (4 * r), (r + 1) * (r + 1));
+                return Eos::ratio(T(0.5) * (dR + dL) * (4 * r), (r + 1) * (r + 1));
             } else if constexpr (LIM == kMinmod) {
-                T const mm = Eos::div(2 * fmin(T(1), r), 1 + r);
+                T const mm = Eos::ratio(2 * fmin(T(1), r), 1 + r);
                 return mm * (dL + dR) * T(0.5);
             } else {
-                return Eos::div(T(0.5) * (dR + dL) * (2 * r), (r * r) + 1);
+                return Eos::ratio(T(0.5) * (dR + dL) * (2 * r), (r * r) + 1);
             }
         }
         return 0;

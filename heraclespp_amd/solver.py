@@ -165,11 +165,11 @@ class HydroSolver:
             if self.nfx:
                 self._cast(self._fxs[self._cur], self._fxs32[self._cur])
             r, m, E = self._cons32[self._cur]
-            check(lib().hclpp_clock_start_f32(c.handle, self.t, float(t_end), self.cfl, int(max_iter), _ptr(r), _ptr(m),
+            check(lib().hclpp_clock_start_f32(c.handle, self.t, self.iter, float(t_end), self.cfl, int(max_iter), _ptr(r), _ptr(m),
                                               _ptr(E)))
         else:
             r, m, E = self._cons[self._cur]
-            check(lib().hclpp_clock_start(c.handle, self.t, float(t_end), self.cfl, int(max_iter), _ptr(r), _ptr(m),
+            check(lib().hclpp_clock_start(c.handle, self.t, self.iter, float(t_end), self.cfl, int(max_iter), _ptr(r), _ptr(m),
                                           _ptr(E)))
 
     def _launch_step(self, a, b):

@@ -617,7 +617,7 @@ public:
     int run(double t_end, int max_iter, int check_every = 64)
     {
         int const it0 = m_iter, cur0 = m_cur;
-        check(hclpp_clock_start(m_ctx, m_t, t_end, m_cfl, max_iter, m_rho[m_cur].data(), m_rhou[m_cur].data(), m_E[m_cur].data()));
+        check(hclpp_clock_start(m_ctx, m_t, m_iter, t_end, m_cfl, max_iter, m_rho[m_cur].data(), m_rhou[m_cur].data(), m_E[m_cur].data()));
         hclpp_clock clk {};
         int launched = 0;
         for (;;) {

@@ -162,9 +162,12 @@ typedef struct {
     int status;
 } hclpp_clock;
 
-/* Start the clock on a ghost-filled conservative state (FP64 arrays). */
-int hclpp_clock_start(hclpp_ctx* ctx, double t0, double t_end, double cfl, int max_iter, const double* rho, const double* rhou,
-                      const double* E);
+/* Start the clock on a ghost-filled conservative state (FP64 arrays) at time t0 and
+ * iteration iter0 (0 for a new run; a restart resumes the iteration count, main.cpp:223-238).
+ * The run stops when t reaches t_end or the iteration count reaches max_iter, counted from
+ * iteration 0 as in main.cpp:382-385. */
+int hclpp_clock_start(hclpp_ctx* ctx, double t0, int iter0, double t_end, double cfl, int max_iter, const double* rho,
+                      const double* rhou, const double* E);
 /* Copy the device clock to the host (synchronises). */
 int hclpp_clock_read(hclpp_ctx* ctx, hclpp_clock* out);
 /* One fused step: reads (rho, rhou, E) ghost-filled, writes the new state,
@@ -189,8 +192,8 @@ int hclpp_step_fx(hclpp_ctx* ctx, const double* rho, const double* rhou, const d
  * which are FP64.  The context must be created with precision =
  * HCLPP_FP32_STORAGE.  Not bit-identical to the FP64 path: the accuracy is
  * stated as an L1 distance to it. */
-int hclpp_clock_start_f32(hclpp_ctx* ctx, double t0, double t_end, double cfl, int max_iter, const float* rho, const float* rhou,
-                          const float* E);
+int hclpp_clock_start_f32(hclpp_ctx* ctx, double t0, int iter0, double t_end, double cfl, int max_iter, const float* rho,
+                          const float* rhou, const float* E);
 int hclpp_step_f32(hclpp_ctx* ctx, const float* rho, const float* rhou, const float* E, float* rho_new, float* rhou_new, float* E_new);
 /* hclpp_step_fx with FP32 storage (the scalar is stored and computed in FP32). */
 int hclpp_step_fx_f32(hclpp_ctx* ctx, const float* rho, const float* rhou, const float* E, const float* fx, float* rho_new,

@@ -1,0 +1,84 @@
+"""Write tests/golden/long_run_digests.json: the CPU oracle's state digests for the configured
+full-length runs, so the GPU box can check N-step bit-identity without minutes of host time.
+
+Runs (the oracle, OpenMP over this host's cores):
+  * 2D Kelvin-Helmholtz 2048^2, VanLeer + HLLC, CFL 0.4, the configured 2000 steps;
+  * 3D Rayleigh-Taylor 128^3 (the 512^3 configuration's physics and boundaries on a grid the
+    oracle finishes in about a minute), Minmod + HLLC, uniform gravity, its 500 steps.
+
+Every 100 steps (and at the last one) it records, per conservative variable (rho, rhou, E over
+the ghost-inclusive arrays, float64 little-endian in the oracle's C order -- component axes
+first, x fastest): the SHA-256 of the bytes, the sum and the sum of absolute values; plus the
+clock (t as a hex float, iteration).  tests/test_gpu_long_runs.py recomputes the same digests
+from the GPU state at the same steps.
+
+Usage: python tests/golden/make_long_run_digests.py [--only kh|rt]
+"""
+
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+OUT = os.path.join(HERE, "long_run_digests.json")
+EVERY = 100
+
+
+def digest(arr: np.ndarray) -> dict:
+    a = np.ascontiguousarray(arr, dtype="<f8")
+    return {"sha256": hashlib.sha256(a.tobytes()).hexdigest(), "sum": float(np.sum(a)),
+            "abs_sum": float(np.sum(np.abs(a))), "shape": list(a.shape)}
+
+
+def state_digest(get, t: float, it: int) -> dict:
+    return {"iter": it, "t": float(t).hex(), "vars": {name: digest(get(name)) for name in ("rho", "rhou", "E")}}
+
+
+def run(name: str, s, nsteps: int) -> dict:
+    from tests.helpers import make_oracle
+
+    o = make_oracle(s)
+    o.init_from_prim(s.rho, s.u, s.P)
+    marks = []
+    t0 = time.perf_counter()
+    while o.iter < nsteps:
+        _, ex = o.step(s.cfl, s.t_end, nsteps)
+        if o.iter % EVERY == 0 or ex or o.iter == nsteps:
+            marks.append(state_digest(o.state, o.t, o.iter))
+            print(f"{name}: step {o.iter} ({time.perf_counter() - t0:.0f} s)", flush=True)
+        if ex:
+            break
+    return {"workload": name, "n": list(s.n[:s.ndim]), "steps": nsteps, "every": EVERY, "marks": marks}
+
+
+def main() -> None:
+    from heraclespp_amd import setups
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", choices=["kh", "rt"], default=None)
+    a = ap.parse_args()
+    out = json.load(open(OUT)) if os.path.exists(OUT) else {}
+    if a.only in (None, "rt"):
+        s = setups.rayleigh_taylor(128, seed=0)
+        out["rayleigh_taylor_128"] = run("rayleigh_taylor_128", s, s.max_iter)
+    if a.only in (None, "kh"):
+        s = setups.kelvin_helmholtz(2048, seed=0)
+        out["kelvin_helmholtz_2048"] = run("kelvin_helmholtz_2048", s, s.max_iter)
+    with open(OUT, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+        f.write("\n")
+    print(OUT)
+
+
+if __name__ == "__main__":
+    main()

@@ -81,7 +81,7 @@ def test_step_dt_matches_oracle_fixed_step(cuda, case):
     Ud = [_t(u) for u in U]
     out = [torch.zeros_like(u) for u in Ud]
     L = _lib.lib()
-    _lib.check(L.hclpp_clock_start(ctx.handle, 0.0, 1e300, 0.4, 1 << 30, *[_ptr(t) for t in Ud]))
+    _lib.check(L.hclpp_clock_start(ctx.handle, 0.0, 0, 1e300, 0.4, 1 << 30, *[_ptr(t) for t in Ud]))
     _lib.check(L.hclpp_step_dt(ctx.handle, dt, *[_ptr(t) for t in Ud + out]))
     clk = _lib.Clock()
     _lib.check(L.hclpp_clock_read(ctx.handle, ctypes.byref(clk)))

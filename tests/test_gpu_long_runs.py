@@ -1,19 +1,28 @@
-"""Full-length runs of the configured workloads against the CPU oracle (opt-in: the oracle needs
-minutes of host time, so these run only with HCLPP_LONG_RUNS=1).
+"""Full-length runs of the configured workloads against the CPU oracle.
 
-* 2D Kelvin-Helmholtz 2048^2, VanLeer + HLLC, CFL 0.4, all 2000 configured steps: the fused HIP
-  step's conservative state is bit-identical to the oracle's after the whole run, and the clock
-  (t, step count) agrees exactly.
-* 3D Rayleigh-Taylor (Minmod + HLLC, uniform gravity, periodic x/y and reflexive z walls) at
-  128^3 for its 500 configured steps: bit-identical in the same way.
+* 2D Kelvin-Helmholtz 2048^2, VanLeer + HLLC, CFL 0.4, all 2000 configured steps, and 3D
+  Rayleigh-Taylor (Minmod + HLLC, uniform gravity, periodic x/y, reflexive z walls) at 128^3 for
+  its 500 configured steps: the oracle's runs take minutes of host time, so their state digests
+  (SHA-256 of every conservative array, ghosts included, every 100 steps, plus the clock) were
+  computed once by tests/golden/make_long_run_digests.py and committed.  The fused HIP step must
+  reproduce every digest: bit-identical states and the same t and step count at every mark.
 * 3D spherical Sedov blast (HLL + VanLeer, reflexive r_min and poles) at 128 x 32 x 64 for 400
-  steps: in the still far field the momenta decay below 1e-290, where the kernels' FP64 division
-  may differ from IEEE division in the last bits (physics.hpp fdiv).  Every value of at least
-  2^-960 in magnitude is bit-identical; the tiny ones agree to a few ulp; the specification's
-  bar, relative L1 <= 1e-12 per conserved variable, holds.
+  steps against the live oracle (about 11 s of host time), compared every 100 steps (the
+  divergence curve is printed).  In the still far field the momenta decay below 1e-290, where
+  the default build's FP64 division may differ from IEEE division in the last bits
+  (physics.hpp fdiv): every value of at least 2^-960 in magnitude must be bit-identical, the
+  tiny ones within a few ulp, and the relative L1 distance per conserved variable <= 1e-12.
+  The same run through the IEEE-division build (libhclpp_hip_ieee.so, loaded in a child
+  process through HCLPP_LIB) must be bit-identical everywhere.
 """
 
+import hashlib
+import json
 import os
+import shutil
+import subprocess
+import sys
+import tempfile
 import time
 
 import numpy as np
@@ -22,72 +31,108 @@ import pytest
 from heraclespp_amd import setups
 from tests.helpers import make_oracle, make_solver, rel_l1
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skipif(os.environ.get("HCLPP_LONG_RUNS") != "1", reason="full-length run: set HCLPP_LONG_RUNS=1"),
-]
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DIGESTS = os.path.join(ROOT, "tests", "golden", "long_run_digests.json")
+IEEE_LIB = os.path.join(ROOT, "heraclespp_amd", "libhclpp_hip_ieee.so")
+NAMES = ("rho", "rhou", "E")
 
 
-def _run_both(s, nsteps, label, exact=True):
-    t0 = time.perf_counter()
+def _sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def _gpu_state(sol):
+    return {"rho": sol.rho.cpu().numpy(), "rhou": sol.rhou.cpu().numpy(), "E": sol.E.cpu().numpy()}
+
+
+def _check_against_digests(key: str, s):
+    ref = json.load(open(DIGESTS))[key]
     sol = make_solver(s)
     sol.init_from_prim(s.rho, s.u, s.P)
-    steps = sol.run(s.t_end, nsteps)
-    t1 = time.perf_counter()
-    print(f"\n[long-run] {label}: {steps} GPU steps in {t1 - t0:.1f} s, t = {sol.t:.9g}", flush=True)
-    o = make_oracle(s)
-    o.init_from_prim(s.rho, s.u, s.P)
-    while o.iter < nsteps:
-        _, ex = o.step(s.cfl, s.t_end, nsteps)
-        if o.iter % 100 == 0:
-            print(f"[long-run] oracle step {o.iter} ({time.perf_counter() - t1:.0f} s)", flush=True)
-        if ex:
-            break
-    assert steps == o.iter
-    assert sol.t == o.t
-    for name, gpu in (("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E)):
-        got, ref = gpu.cpu().numpy(), o.state(name)
-        if exact:
-            np.testing.assert_array_equal(got, ref, err_msg=name)
-            continue
-        big = np.abs(ref) >= 2.0 ** -960
-        np.testing.assert_array_equal(got[big], ref[big], err_msg=name)
-        d = np.abs(got[~big] - ref[~big])
-        assert np.all(d <= 4 * np.spacing(np.abs(ref[~big])) + 2.0 ** -1060), name
-        assert rel_l1(got, ref) <= 1e-12, name
-        print(f"[long-run] {name}: {int(np.count_nonzero(got != ref))} values below 2^-960 differ, max |diff| {d.max() if d.size else 0.0:.3e}",
-              flush=True)
-    return steps
+    t0 = time.perf_counter()
+    for mark in ref["marks"]:
+        sol.run(s.t_end, mark["iter"])
+        st = _gpu_state(sol)
+        same = {n: _sha(st[n]) == mark["vars"][n]["sha256"] for n in NAMES}
+        # divergence curve: relative distance of the totals (the digests carry no arrays)
+        drift = {n: abs(float(np.sum(st[n])) - mark["vars"][n]["sum"]) / max(mark["vars"][n]["abs_sum"], 1e-300)
+                 for n in NAMES}
+        print(f"[long-run] {key} step {sol.iter}: identical={all(same.values())} t_gpu={sol.t.hex()} "
+              f"t_oracle={mark['t']} total drift {max(drift.values()):.3e} ({time.perf_counter() - t0:.1f} s)", flush=True)
+        assert sol.iter == mark["iter"], (sol.iter, mark["iter"])
+        assert sol.t == float.fromhex(mark["t"]), (sol.t, mark["t"])
+        for n in NAMES:
+            assert list(st[n].shape) == mark["vars"][n]["shape"], n
+            assert same[n], f"{key}: {n} differs from the oracle at step {mark['iter']}"
+    return sol.iter
 
 
 def test_rayleigh_taylor_128_full_run_is_bit_identical(cuda):
     s = setups.rayleigh_taylor(128, seed=0)
-    assert _run_both(s, s.max_iter, "RT 128^3") == 500
-
-
-def test_sedov_spherical_long_run_matches_oracle(cuda):
-    s = setups.sedov(128, 32, 64, t_end=1.0)
-    assert _run_both(s, 400, "Sedov 128x32x64", exact=False) == 400
+    assert _check_against_digests("rayleigh_taylor_128", s) == s.max_iter == 500
 
 
 def test_kelvin_helmholtz_2048_full_run_is_bit_identical(cuda):
     s = setups.kelvin_helmholtz(2048, seed=0)
-    assert s.max_iter == 2000
-    t0 = time.perf_counter()
-    sol = make_solver(s)
-    sol.init_from_prim(s.rho, s.u, s.P)
-    steps = sol.run(s.t_end, s.max_iter)
-    t1 = time.perf_counter()
-    print(f"\n[long-run] KH 2048^2: {steps} GPU steps in {t1 - t0:.1f} s, t = {sol.t:.9g}", flush=True)
+    assert _check_against_digests("kelvin_helmholtz_2048", s) == s.max_iter == 2000
+
+
+SEDOV = dict(nr=128, nth=32, nph=64, t_end=1.0)
+SEDOV_STEPS, EVERY = 400, 100
+
+
+def _compare_sedov(step, got: dict, o, exact: bool, label: str):
+    worst = 0.0
+    for n in NAMES:
+        g, r = got[n], o.state(n)
+        d = np.abs(g - r)
+        worst = max(worst, float(np.max(d)))
+        if exact:
+            np.testing.assert_array_equal(g, r, err_msg=f"{label} {n} at step {step}")
+            continue
+        big = np.abs(r) >= 2.0 ** -960
+        np.testing.assert_array_equal(g[big], r[big], err_msg=f"{label} {n} at step {step}")
+        small = d[~big]
+        assert np.all(small <= 4 * np.spacing(np.abs(r[~big])) + 2.0 ** -1060), f"{label} {n} at step {step}"
+        assert rel_l1(g, r) <= 1e-12, f"{label} {n} at step {step}"
+    print(f"[long-run] {label} step {step}: max |gpu - oracle| = {worst:.3e}", flush=True)
+
+
+def test_sedov_spherical_long_run_matches_oracle(cuda):
+    s = setups.sedov(**SEDOV)
     o = make_oracle(s)
     o.init_from_prim(s.rho, s.u, s.P)
-    while o.iter < s.max_iter:
-        _, ex = o.step(s.cfl, s.t_end, s.max_iter)
-        if o.iter % 100 == 0:
-            print(f"[long-run] oracle step {o.iter} ({time.perf_counter() - t1:.0f} s)", flush=True)
-        if ex:
-            break
-    assert steps == o.iter == 2000
-    assert sol.t == o.t
-    for name, gpu in (("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E)):
-        np.testing.assert_array_equal(gpu.cpu().numpy(), o.state(name), err_msg=name)
+    sol = make_solver(s)
+    sol.init_from_prim(s.rho, s.u, s.P)
+    # the IEEE-division build, in a child process (one library per process), runs concurrently
+    # with the oracle and writes its state every EVERY steps
+    tmp = tempfile.mkdtemp(prefix="hclpp_ieee_")
+    env = dict(os.environ, HCLPP_LIB=IEEE_LIB)
+    child = subprocess.Popen([sys.executable, "-u", "-m", "tests.ieee_division_run", tmp, str(SEDOV_STEPS), str(EVERY)],
+                             cwd=ROOT, env=env)
+    try:
+        for step in range(EVERY, SEDOV_STEPS + 1, EVERY):
+            while o.iter < step:
+                o.step(s.cfl, s.t_end, SEDOV_STEPS)
+            sol.run(s.t_end, step)
+            assert sol.iter == o.iter == step
+            assert sol.t == o.t
+            _compare_sedov(step, _gpu_state(sol), o, exact=False, label="Sedov 128x32x64 (default build)")
+            # the IEEE build's state at this step (the child may still be running)
+            path = os.path.join(tmp, f"state_{step}.npz")
+            deadline = time.time() + 120
+            while not os.path.exists(path):
+                assert child.poll() in (None, 0), f"IEEE-division run failed with status {child.returncode}"
+                assert time.time() < deadline, "IEEE-division run: no state written"
+                time.sleep(0.2)
+            z = np.load(path)
+            assert int(z["iter"]) == step and float(z["t"]) == o.t
+            _compare_sedov(step, {n: z[n] for n in NAMES}, o, exact=True, label="Sedov 128x32x64 (IEEE division)")
+        assert child.wait(timeout=120) == 0
+    finally:
+        if child.poll() is None:
+            child.kill()
+            child.wait()
+        shutil.rmtree(tmp, ignore_errors=True)
