@@ -37,7 +37,7 @@ EXPORTS = [
     "hclpp_clock_start_f32", "hclpp_step_f32", "hclpp_step_fx_f32",
     "hclpp_cast_f64_to_f32", "hclpp_cast_f32_to_f64", "hclpp_step_dt", "hclpp_clock_record_dt", "hclpp_step_adjoint",
     "hclpp_steps_dt", "hclpp_steps_adjoint", "hclpp_pressure_fix", "hclpp_internal_mass_gravity", "hclpp_gravity_table",
-    "hclpp_profile_begin", "hclpp_profile_end", "hclpp_fused_plan",
+    "hclpp_profile_begin", "hclpp_profile_end", "hclpp_fused_plan", "hclpp_time_step_noalloc",
 ]
 
 
@@ -116,6 +116,7 @@ def lib() -> ctypes.CDLL:
         "hclpp_hydro_reconstruction": [vp, ip, rp, d] + [dp] * 10,
         "hclpp_godunov": [vp, ip, rp, d] + [dp] * 12,
         "hclpp_time_step": [vp, rp, dp, dp, dp, ctypes.POINTER(d)],
+        "hclpp_time_step_noalloc": [vp, rp, dp, dp, dp, dp],
         "hclpp_min_internal_energy": [vp, rp, dp, dp, dp, ctypes.POINTER(d)],
         "hclpp_integrate": [vp, rp, dp, ctypes.POINTER(d)],
         "hclpp_fill_ghosts": [vp, dp, dp, dp, dp],

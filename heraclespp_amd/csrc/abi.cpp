@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -65,6 +66,18 @@ namespace {
 
 thread_local std::string g_error;
 
+// ---- scratch of hclpp_time_step_noalloc: the workgroup partials of its reduction live in this
+// static device array, so that entry point calls no allocating routine.  Every GPU the library
+// runs on holds its own copy, at its own address, until the process ends.  In host code the
+// array's name is not a device address: kernels and copies get the address hipGetSymbolAddress
+// returns for the current device, looked up once per device when a context is created there
+// (noalloc_scratch_setup) and kept in g_scratch.
+constexpr int64_t kNoallocScratchDoubles = int64_t(1) << 20;  // 8 MiB: one partial per 64 x 4 workgroup
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+std::mutex g_scratch_mutex;
+double* g_scratch[kMaxDevices] = {};
+
 int fail(int code, const std::string& msg)
 {
     g_error = msg;
@@ -87,6 +100,26 @@ int hip_fail(hipError_t e, const char* what)
         hipError_t const e_ = hipGetLastError();            \
         if (e_ != hipSuccess) return hip_fail(e_, what);    \
     } while (0)
+
+// Look up this device's copy of `scratch` once (the device must be current).
+int noalloc_scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fail(HCLPP_ERR_UNSUPPORTED, "device index beyond the no-allocation scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[dev]) return HCLPP_OK;
+    void* ptr = nullptr;
+    hipError_t const e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch[dev] = static_cast<double*>(ptr);
+    return HCLPP_OK;
+}
+
+double* noalloc_scratch(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch[dev];
+}
 
 Range3 to_range(const hclpp_range* r)
 {
@@ -257,6 +290,10 @@ int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out)
     };
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) return cleanup(hip_fail(e, "hipGetDevice"));
+    {
+        int const rc = noalloc_scratch_setup(c->device);
+        if (rc != HCLPP_OK) return cleanup(rc);
+    }
     e = hipMalloc(&c->tables, sizeof(double) * h.size());
     if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(tables)"));
     e = hipMemcpy(c->tables, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice);
@@ -468,6 +505,27 @@ int hclpp_time_step(hclpp_ctx* c, const hclpp_range* r, const double* rho, const
     rc = read_scalar(c, &inv);
     if (rc) return rc;
     *dt_out = 1 / inv;  // time_step.hpp:81
+    return HCLPP_OK;
+}
+
+int hclpp_time_step_noalloc(hclpp_ctx* c, const hclpp_range* r, const double* rho, const double* u, const double* P, double* dt_dev)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    int rc = check_range(c, r, 0);
+    if (rc) return rc;
+    Range3 const R = to_range(r);
+    int64_t const nb = reduce_blocks(R);
+    if (nb > kNoallocScratchDoubles)
+        return fail(HCLPP_ERR_ARG, "hclpp_time_step_noalloc: the range needs " + std::to_string(nb) + " workgroup partials; the "
+                                       "no-allocation scratch holds " + std::to_string(kNoallocScratchDoubles) +
+                                       " (64 x 4 cells each): use hclpp_time_step");
+    if (nb == 0) return fail(HCLPP_ERR_ARG, "empty range");
+    if (!rho || !u || !P || !dt_dev) return fail(HCLPP_ERR_ARG, "null argument");
+    double* const ptr = noalloc_scratch(c->device);
+    if (!ptr) return fail(HCLPP_ERR_STATE, "no-allocation scratch not set up on the context's device");
+    // every partial the fold reads is written by this launch: nothing needs clearing
+    launch_time_step_inverse(c->cfg.ndim, c->cfg.geom, c->G, R, c->eos, rho, u, P, ptr, dt_dev, c->stream);
+    HCLPP_LAUNCHED("time_step (no allocation)");
     return HCLPP_OK;
 }
 

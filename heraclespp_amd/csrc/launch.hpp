@@ -101,6 +101,9 @@ void launch_min_internal_energy(int nd, const GridDev& G, const Range3& r, const
 void launch_integrate(const GridDev& G, int nd, int geom, const Range3& r, const double* var, double* partials, double* out, hipStream_t s);
 // Fold n partials (max) into out[0] in a fixed order.
 void launch_fold_max(const double* partials, int64_t n, double* out, hipStream_t s);
+// launch_time_step_rate folding to the reciprocal of the maximum (time_step.hpp:81) in device memory
+void launch_time_step_inverse(int nd, int geom, const GridDev& G, const Range3& r, const PerfectGas& eos, const double* rho, const double* u,
+                              const double* P, double* partials, double* dt_out, hipStream_t s);
 // hydro/pressure_fix.hpp:19-172 (E_new updated in place)
 void launch_pressure_fix(int nd, int geom, const GridDev& G, const Range3& r, const PerfectGas& eos, double dt, double eps, const double* rho,
                          const double* rhou, const double* E, const double* rho_rec, const double* rhou_rec, const double* E_rec,

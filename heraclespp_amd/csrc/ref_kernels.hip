@@ -553,6 +553,24 @@ void launch_time_step_cons(int nd, int geom, const GridDev& G, const Range3& r, 
     hipLaunchKernelGGL(k_fold<kMax>, dim3(1), dim3(1024), 0, s, partials, reduce_blocks(r), out);
 }
 
+// k_fold<kMax> writing the reciprocal of the maximum: time_step.hpp:81 returns 1 / max
+__global__ __launch_bounds__(1024) void k_fold_max_inverse(const double* __restrict__ partials, int64_t n, double* __restrict__ out)
+{
+    __shared__ double lds[16];
+    double v = reduce_identity<kMax>();
+    for (int64_t t = threadIdx.x; t < n; t += 1024) v = reduce_combine<kMax>(v, partials[t]);
+    v = block_reduce<kMax>(v, lds, threadIdx.x, 1024);
+    if (threadIdx.x == 0) out[0] = 1 / v;
+}
+
+void launch_time_step_inverse(int nd, int geom, const GridDev& G, const Range3& r, const PerfectGas& eos, const double* rho, const double* u,
+                              const double* P, double* partials, double* dt_out, hipStream_t s)
+{
+    HCLPP_GEOM_SWITCH(geom, HCLPP_ND_SWITCH(nd, hipLaunchKernelGGL((k_time_step<ND, GEOM>), range_grid(r), dim3(kBX, kBY), 0, s, G, r, eos, rho,
+                                                                   u, P, partials)));
+    hipLaunchKernelGGL(k_fold_max_inverse, dim3(1), dim3(1024), 0, s, partials, reduce_blocks(r), dt_out);
+}
+
 void launch_fold_max(const double* partials, int64_t n, double* out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fold<kMax>, dim3(1), dim3(1024), 0, s, partials, n, out);

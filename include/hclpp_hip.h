@@ -117,6 +117,16 @@ int hclpp_godunov(hclpp_ctx* ctx, int riemann, const hclpp_range* range, double 
 /* time_step<EoS>: hydro/time_step.hpp:35-82; *dt_out = 1 / max(sum_d (|u_d|+c)/dx_geom_d)
  * (the caller multiplies by cfl, as main.cpp:355 does).  Synchronises. */
 int hclpp_time_step(hclpp_ctx* ctx, const hclpp_range* range, const double* rho, const double* u, const double* P, double* dt_out);
+/* hclpp_time_step for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same kernels and the same result, bit for bit, written to the device double
+ * *dt_dev on the context's stream; no allocation and no synchronisation.  The workgroup
+ * partials live in a static device array of 2^20 doubles (8 MiB, held by each GPU the library
+ * runs on until the process ends): a range needing more than 2^20 workgroups of 64 x 4 cells
+ * (about 268 M cells) fails with HCLPP_ERR_ARG.  Every call on one GPU shares that array, so
+ * calls that overlap in time on one GPU (other streams or threads) race. */
+int hclpp_time_step_noalloc(hclpp_ctx* ctx, const hclpp_range* range, const double* rho, const double* u, const double* P,
+                            double* dt_dev);
 /* minimum_internal_energy: analysis/internal_energy.cpp:20-45.  Synchronises. */
 int hclpp_min_internal_energy(hclpp_ctx* ctx, const hclpp_range* range, const double* rho, const double* rhou, const double* E,
                               double* out);
