@@ -32,6 +32,7 @@ EXPORTS = [
     "hclpp_last_error", "hclpp_ctx_create", "hclpp_ctx_destroy", "hclpp_set_stream", "hclpp_range_with_ghosts",
     "hclpp_limiter_from_name", "hclpp_riemann_from_name", "hclpp_bc_from_name",
     "hclpp_conv_prim_to_cons", "hclpp_conv_cons_to_prim", "hclpp_face_reconstruction", "hclpp_hancock_extrapolation",
+    "hclpp_face_prim_to_cons",
     "hclpp_hydro_reconstruction", "hclpp_godunov", "hclpp_time_step", "hclpp_min_internal_energy", "hclpp_integrate",
     "hclpp_fill_ghosts", "hclpp_clock_start", "hclpp_clock_read", "hclpp_step", "hclpp_step_fx",
     "hclpp_clock_start_f32", "hclpp_step_f32", "hclpp_step_fx_f32",
@@ -114,6 +115,7 @@ def lib() -> ctypes.CDLL:
         "hclpp_face_reconstruction": [vp, ip, rp, dp, dp],
         "hclpp_hancock_extrapolation": [vp, rp, d, dp, dp, dp, dp, dp, dp],
         "hclpp_hydro_reconstruction": [vp, ip, rp, d] + [dp] * 10,
+        "hclpp_face_prim_to_cons": [vp, rp] + [dp] * 5,
         "hclpp_godunov": [vp, ip, rp, d] + [dp] * 12,
         "hclpp_time_step": [vp, rp, dp, dp, dp, ctypes.POINTER(d)],
         "hclpp_time_step_noalloc": [vp, rp, dp, dp, dp, dp],

@@ -454,6 +454,18 @@ int hclpp_hancock_extrapolation(hclpp_ctx* c, const hclpp_range* r, double dt, c
     return hancock_impl(c, r, dt, u_rec, P_rec, rho_rec, rhou_rec, E_rec, fx_rec);
 }
 
+int hclpp_face_prim_to_cons(hclpp_ctx* c, const hclpp_range* r, const double* rho_rec, const double* u_rec, const double* P_rec, double* rhou_rec,
+                            double* E_rec)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    int const rc = check_range(c, r, 0);
+    if (rc) return rc;
+    if (!rho_rec || !u_rec || !P_rec || !rhou_rec || !E_rec) return fail(HCLPP_ERR_ARG, "null array");
+    launch_face_prim_to_cons(c->cfg.ndim, c->G, to_range(r), c->eos, rho_rec, u_rec, P_rec, rhou_rec, E_rec, c->stream);
+    HCLPP_LAUNCHED("conv_prim_to_cons (face states)");
+    return HCLPP_OK;
+}
+
 int hclpp_hydro_reconstruction(hclpp_ctx* c, int limiter, const hclpp_range* r, double dt, const double* rho, const double* u, const double* P,
                                const double* fx, double* rho_rec, double* rhou_rec, double* E_rec, double* fx_rec, double* P_rec,
                                double* u_rec)

@@ -170,6 +170,13 @@ class Grid:
     def zeros(self, *lead, dtype=torch.float64, device="cuda"):
         return torch.zeros(tuple(lead) + self.shape3, dtype=dtype, device=device)
 
+    def geometry_context(self, bc=None, bc_order=None, nfx: int = 0) -> "Context":
+        """The device copy of this grid's tables for the operators that read only the grid and the arrays
+        they are given (face reconstruction, integrate, minimum_internal_energy, the ghost fill): a context
+        whose equation of state, gravity and scheme fields those operators never read (they hold valid
+        neutral values, as the context checks them)."""
+        return self.context(PerfectGas(1.4, 1.0), bc=bc, bc_order=bc_order, nfx=nfx)
+
     def context(self, eos: PerfectGas, gravity=None, bc=None, bc_order=None, nfx: int = 0, limiter="VanLeer",
                 riemann="HLL", precision="fp64") -> "Context":
         key = (eos.adiabatic_index(), eos.mean_molecular_weight(), _gravity_key(gravity), tuple(bc or ()),
@@ -375,8 +382,8 @@ class LimitedLinearReconstruction(IFaceReconstruction):
         self.slope_limiter = slope_limiter
         self._lim = _lib.LIMITER[slope_limiter]
 
-    def execute(self, range_: Range, grid: Grid, var, var_rec, eos: PerfectGas | None = None):
-        c = grid.context(eos or PerfectGas(1.4, 1.0))
+    def execute(self, range_: Range, grid: Grid, var, var_rec):
+        c = grid.geometry_context()
         check(lib().hclpp_face_reconstruction(c.handle, self._lim, ctypes.byref(range_._c()), c._chk(var, (), "var"),
                                               c._chk(var_rec, (grid.ndim, 2), "var_rec")))
 
@@ -421,24 +428,50 @@ class IHydroReconstruction:
 
 
 class MUSCLHancockHydroReconstruction(IHydroReconstruction):
-    """hydro/hydro_reconstruction.hpp:61-130: P_rec and u_rec are the operator's own scratch views."""
+    """hydro/hydro_reconstruction.hpp:61-130: P_rec and u_rec are the operator's own scratch views.
+
+    execute composes the injected strategies as the reference does: the face reconstruction of rho, each
+    velocity component, P and each passive scalar, the prim -> cons conversion of every face state, then the
+    extrapolation step.  When both strategies are the library's own (LimitedLinearReconstruction and
+    HancockExtrapolationReconstruction with this operator's EoS), the same sequence runs as one C-ABI call
+    (hclpp_hydro_reconstruction), which launches the same kernels."""
 
     def __init__(self, face_reconstruction: IFaceReconstruction, hancock_reconstruction: IExtrapolationReconstruction,
                  eos: PerfectGas, P_rec, u_rec):
-        if not isinstance(face_reconstruction, LimitedLinearReconstruction):
-            raise TypeError("face_reconstruction must come from factory_face_reconstruction")
-        if not isinstance(hancock_reconstruction, HancockExtrapolationReconstruction):
-            raise TypeError("hancock_reconstruction must be a HancockExtrapolationReconstruction")
         self.face_reconstruction = face_reconstruction
         self.hancock_reconstruction = hancock_reconstruction
         self.eos = eos
         self.P_rec = P_rec
         self.u_rec = u_rec
 
+    def _library_strategies(self) -> bool:
+        h = self.hancock_reconstruction
+        return (type(self.face_reconstruction) is LimitedLinearReconstruction
+                and type(h) is HancockExtrapolationReconstruction
+                and h.eos.adiabatic_index() == self.eos.adiabatic_index()
+                and h.eos.mean_molecular_weight() == self.eos.mean_molecular_weight())
+
     def execute(self, range_, grid, gravity, dt, rho, u, P, fx, rho_rec, rhou_rec, E_rec, fx_rec):
         nd = grid.ndim
         nfx = 0 if fx is None else fx.shape[0]
         c = grid.context(self.eos, gravity, nfx=nfx)
+        if not self._library_strategies():
+            face = self.face_reconstruction
+            face.execute(range_, grid, rho, rho_rec)  # hydro_reconstruction.hpp:99-109
+            for d in range(nd):
+                face.execute(range_, grid, u[d], self.u_rec[d])
+            face.execute(range_, grid, P, self.P_rec)
+            for f in range(nfx):
+                face.execute(range_, grid, fx[f], fx_rec[f])
+            # :111-126, conv_prim_to_cons of every face state
+            check(lib().hclpp_face_prim_to_cons(
+                c.handle, ctypes.byref(range_._c()), c._chk(rho_rec, (nd, 2), "rho_rec"),
+                c._chk(self.u_rec, (nd, nd, 2), "u_rec"), c._chk(self.P_rec, (nd, 2), "P_rec"),
+                c._chk(rhou_rec, (nd, nd, 2), "rhou_rec"), c._chk(E_rec, (nd, 2), "E_rec")))
+            # :128
+            self.hancock_reconstruction.execute(range_, grid, gravity, dt, self.u_rec, self.P_rec, rho_rec, rhou_rec,
+                                                E_rec, fx_rec)
+            return
         check(lib().hclpp_hydro_reconstruction(
             c.handle, self.face_reconstruction._lim, ctypes.byref(range_._c()), float(dt), c._chk(rho, (), "rho"),
             c._chk(u, (nd,), "u"), c._chk(P, (), "P"), c._fx(fx, (nfx,), "fx"), c._chk(rho_rec, (nd, 2), "rho_rec"),
@@ -496,7 +529,7 @@ def time_step(range_: Range, eos: PerfectGas, grid: Grid, rho, u, P) -> float:
 
 def minimum_internal_energy(range_: Range, grid: Grid, rho, rhou, E) -> float:
     """analysis/internal_energy.cpp:20-45."""
-    c = grid.context(PerfectGas(1.4, 1.0))
+    c = grid.geometry_context()
     out = ctypes.c_double()
     check(lib().hclpp_min_internal_energy(c.handle, ctypes.byref(range_._c()), c._chk(rho, (), "rho"),
                                           c._chk(rhou, (grid.ndim,), "rhou"), c._chk(E, (), "E"), ctypes.byref(out)))
@@ -505,7 +538,7 @@ def minimum_internal_energy(range_: Range, grid: Grid, rho, rhou, E) -> float:
 
 def integrate(range_: Range, grid: Grid, var) -> float:
     """analysis/integration.cpp:22-37."""
-    c = grid.context(PerfectGas(1.4, 1.0))
+    c = grid.geometry_context()
     out = ctypes.c_double()
     check(lib().hclpp_integrate(c.handle, ctypes.byref(range_._c()), c._chk(var, (), "var"), ctypes.byref(out)))
     return out.value
@@ -520,9 +553,9 @@ class DistributedBoundaryCondition:
     bc: list
     bc_order: list | None = None
 
-    def __call__(self, rho, rhou, E, fx=None, eos: PerfectGas | None = None):
+    def __call__(self, rho, rhou, E, fx=None):
         nfx = 0 if fx is None else fx.shape[0]
-        c = self.grid.context(eos or PerfectGas(1.4, 1.0), bc=self.bc, bc_order=self.bc_order, nfx=nfx)
+        c = self.grid.geometry_context(bc=self.bc, bc_order=self.bc_order, nfx=nfx)
         nd = self.grid.ndim
         check(lib().hclpp_fill_ghosts(c.handle, c._chk(rho, (), "rho"), c._chk(rhou, (nd,), "rhou"),
                                       c._chk(E, (), "E"), c._fx(fx, (nfx,), "fx")))

@@ -9,10 +9,12 @@
 // throws std::domain_error on invalid parameters, Range::with_ghosts throws on
 // NgEff > Ng, and any non-zero C-ABI status becomes std::runtime_error.
 //
-// Views: any type with data() and extent_int(i) in Kokkos::LayoutLeft order
-// works (Kokkos::View<double***, LayoutLeft> included); DeviceView below is
-// the owning stand-in used when no Kokkos is present.  All views hold device
-// memory.  Citations are to the reference tree (src/...).
+// Views: DeviceView below is the owning stand-in for Kokkos::View<..., LayoutLeft>
+// (extent 0 fastest), with the reference's aliases KV_double_Nd and the read-only
+// KV_cdouble_Nd (utils/kokkos_shortcut.hpp:22-48); a writable view converts to a
+// read-only one, and slice(i) is Kokkos::subview(v, ALL, ..., ALL, i) of the slowest
+// extent.  The free functions (conv_*) take any type with data() and extent_int(i).
+// All views hold device memory.  Citations are to the reference tree (src/...).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,11 +44,14 @@ inline void check_hip(hipError_t e, char const* what)
 }
 
 // ------------------------------------------------------------ views
-// Owning LayoutLeft device array: extent(0) fastest.  Copies share ownership.
-template <int Rank>
+// Owning LayoutLeft device array: extent(0) fastest.  Copies share ownership; T = const double
+// gives the read-only view a writable one converts to.
+template <int Rank, class T = double>
 class DeviceView
 {
-    std::shared_ptr<double> m_data;
+    template <int, class>
+    friend class DeviceView;
+    std::shared_ptr<T> m_data;
     std::array<int, Rank> m_ext {};
     std::string m_label;
 
@@ -57,6 +62,7 @@ public:
     explicit DeviceView(std::string label, I... ext) : m_ext {static_cast<int>(ext)...}, m_label(std::move(label))
     {
         static_assert(sizeof...(I) == Rank, "one extent per rank");
+        static_assert(!std::is_const_v<T>, "allocate a writable view, then convert it");
         double* p = nullptr;
         size_t const n = size();
         if (n > 0) {
@@ -68,7 +74,26 @@ public:
         });
     }
 
-    double* data() const { return m_data.get(); }
+    // read-only view of a writable one (KV_double_Nd -> KV_cdouble_Nd)
+    template <class U, class = std::enable_if_t<std::is_same_v<T, U const> && !std::is_same_v<T, U>>>
+    DeviceView(DeviceView<Rank, U> const& o) : m_data(o.m_data), m_ext(o.m_ext), m_label(o.m_label)
+    {
+    }
+
+    // Kokkos::subview(v, ALL, ..., ALL, i): index i of the slowest extent, sharing ownership
+    DeviceView<Rank - 1, T> slice(int i) const
+    {
+        static_assert(Rank >= 2, "slice of a rank >= 2 view");
+        if (i < 0 || i >= m_ext[Rank - 1]) throw std::out_of_range("slice index out of range in " + m_label);
+        size_t const stride = m_ext[Rank - 1] > 0 ? size() / static_cast<size_t>(m_ext[Rank - 1]) : 0;
+        DeviceView<Rank - 1, T> s;
+        s.m_data = std::shared_ptr<T>(m_data, data() + stride * static_cast<size_t>(i));
+        for (int r = 0; r < Rank - 1; ++r) s.m_ext[r] = m_ext[r];
+        s.m_label = m_label + "[" + std::to_string(i) + "]";
+        return s;
+    }
+
+    T* data() const { return m_data.get(); }
     int extent_int(int i) const { return i < Rank ? m_ext[i] : 1; }
     size_t extent(int i) const { return static_cast<size_t>(extent_int(i)); }
     size_t size() const
@@ -81,6 +106,7 @@ public:
 
     void copy_from_host(std::vector<double> const& h) const
     {
+        static_assert(!std::is_const_v<T>, "read-only view");
         if (h.size() != size()) throw std::runtime_error("copy_from_host: size mismatch for " + m_label);
         check_hip(hipMemcpy(data(), h.data(), size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
     }
@@ -96,6 +122,10 @@ using KV_double_3d = DeviceView<3>;
 using KV_double_4d = DeviceView<4>;
 using KV_double_5d = DeviceView<5>;
 using KV_double_6d = DeviceView<6>;
+using KV_cdouble_3d = DeviceView<3, const double>;
+using KV_cdouble_4d = DeviceView<4, const double>;
+using KV_cdouble_5d = DeviceView<5, const double>;
+using KV_cdouble_6d = DeviceView<6, const double>;
 
 // ------------------------------------------------------------ thermodynamics/perfect_gas.hpp:17-90
 class PerfectGas
@@ -281,6 +311,15 @@ public:
         return h;
     }
 
+    // The device copy of this grid's tables for the operators that read only the grid and the
+    // arrays they are given (face reconstruction, integrate, minimum_internal_energy, the ghost
+    // fill): a context whose equation of state, gravity and scheme fields those operators never
+    // read (they are set to valid neutral values, as hclpp_ctx_create checks them).
+    hclpp_ctx* geometry_context(std::array<int, 6> bc = {0, 0, 0, 0, 0, 0}, std::array<int, 6> const* order = nullptr, int nfx = 0) const
+    {
+        return context(PerfectGas(kNeutralGamma, 1.0), HCLPP_GRAV_UNIFORM, {0.0, 0.0, 0.0}, 0.0, bc, order ? *order : default_order(), nfx);
+    }
+
     std::array<int, 6> default_order() const
     {
         std::array<int, 6> o {0, 0, 0, 0, 0, 0};
@@ -292,6 +331,7 @@ public:
     long long& gravity_generation(hclpp_ctx* c) const { return (*m_gen)[c]; }
 
 private:
+    static constexpr double kNeutralGamma = 1.4;
     std::shared_ptr<std::map<CtxKey, std::shared_ptr<hclpp_ctx>>> m_ctx = std::make_shared<std::map<CtxKey, std::shared_ptr<hclpp_ctx>>>();
     std::shared_ptr<std::map<hclpp_ctx*, long long>> m_gen = std::make_shared<std::map<hclpp_ctx*, long long>>();
 };
@@ -353,23 +393,26 @@ class IFaceReconstruction
 {
 public:
     virtual ~IFaceReconstruction() noexcept = default;
-    virtual void execute(Range const& range, Grid const& grid, KV_double_3d const& var, KV_double_5d const& var_rec) const = 0;
-    virtual int limiter() const noexcept = 0;
+    //! @param[in] range output iteration range
+    //! @param[in] grid provides grid information
+    //! @param[in] var cell values
+    //! @param[out] var_rec reconstructed values at interfaces
+    virtual void execute(Range const& range, Grid const& grid, KV_cdouble_3d const& var, KV_double_5d const& var_rec) const = 0;
 };
 
-// hydro/limited_linear_reconstruction.hpp:25-66
+// hydro/limited_linear_reconstruction.hpp:25-66; the slope limiter is one of the HCLPP_LIM_* kinds
 class LimitedLinearReconstruction : public IFaceReconstruction
 {
     int m_lim;
 
 public:
-    explicit LimitedLinearReconstruction(int lim) : m_lim(lim) {}
-    void execute(Range const& range, Grid const& grid, KV_double_3d const& var, KV_double_5d const& var_rec) const final
+    explicit LimitedLinearReconstruction(int slope_limiter) : m_lim(slope_limiter) {}
+    void execute(Range const& range, Grid const& grid, KV_cdouble_3d const& var, KV_double_5d const& var_rec) const final
     {
         hclpp_range const r = range.cells();
-        check(hclpp_face_reconstruction(detail::ctx_for(grid, PerfectGas(1.4, 1.0)), m_lim, &r, var.data(), var_rec.data()));
+        check(hclpp_face_reconstruction(grid.geometry_context(), m_lim, &r, var.data(), var_rec.data()));
     }
-    int limiter() const noexcept final { return m_lim; }
+    int slope_limiter() const noexcept { return m_lim; }
 };
 
 // hydro/face_reconstruction.cpp:31-50
@@ -386,8 +429,8 @@ class IExtrapolationReconstruction
 {
 public:
     virtual ~IExtrapolationReconstruction() noexcept = default;
-    virtual void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt_reconstruction, KV_double_6d const& u_rec,
-                         KV_double_5d const& P_rec, KV_double_5d const& rho_rec, KV_double_6d const& rhou_rec, KV_double_5d const& E_rec,
+    virtual void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt_reconstruction, KV_cdouble_6d const& u_rec,
+                         KV_cdouble_5d const& P_rec, KV_double_5d const& rho_rec, KV_double_6d const& rhou_rec, KV_double_5d const& E_rec,
                          KV_double_6d const& fx_rec) const = 0;
 };
 
@@ -399,13 +442,14 @@ class HancockExtrapolationReconstruction : public IExtrapolationReconstruction<G
 
 public:
     explicit HancockExtrapolationReconstruction(PerfectGas const& eos) : m_eos(eos) {}
-    void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt, KV_double_6d const& u_rec, KV_double_5d const& P_rec,
+    void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt, KV_cdouble_6d const& u_rec, KV_cdouble_5d const& P_rec,
                  KV_double_5d const& rho_rec, KV_double_6d const& rhou_rec, KV_double_5d const& E_rec, KV_double_6d const& fx_rec) const final
     {
         hclpp_range const r = range.cells();
         check(hclpp_hancock_extrapolation(detail::ctx_for(grid, m_eos, gravity, detail::nfx_of(fx_rec, 5)), &r, dt, u_rec.data(), P_rec.data(),
                                           rho_rec.data(), rhou_rec.data(), E_rec.data(), fx_rec.data()));
     }
+    PerfectGas const& eos() const noexcept { return m_eos; }
 };
 
 // ------------------------------------------------------------ hydro/hydro_reconstruction.hpp:30-59
@@ -414,12 +458,17 @@ class IHydroReconstruction
 {
 public:
     virtual ~IHydroReconstruction() noexcept = default;
-    virtual void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt, KV_double_3d const& rho, KV_double_4d const& u,
-                         KV_double_3d const& P, KV_double_4d const& fx, KV_double_5d const& rho_rec, KV_double_6d const& rhou_rec,
+    virtual void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt, KV_cdouble_3d const& rho, KV_cdouble_4d const& u,
+                         KV_cdouble_3d const& P, KV_cdouble_4d const& fx, KV_double_5d const& rho_rec, KV_double_6d const& rhou_rec,
                          KV_double_5d const& E_rec, KV_double_6d const& fx_rec) const = 0;
 };
 
-// :61-130 -- P_rec and u_rec are the operator's own scratch views
+// :61-130 -- P_rec and u_rec are the operator's own scratch views.  execute composes the injected
+// strategies exactly as the reference does: the face reconstruction of rho, each velocity
+// component, P and each passive scalar, the prim -> cons conversion of every face state, then the
+// extrapolation step.  When both strategies are the library's own (LimitedLinearReconstruction and
+// HancockExtrapolationReconstruction with this operator's EoS), the same sequence runs as one
+// C-ABI call (hclpp_hydro_reconstruction), which launches the same kernels.
 template <class Gravity>
 class MUSCLHancockHydroReconstruction : public IHydroReconstruction<Gravity>
 {
@@ -440,14 +489,31 @@ public:
         , m_u_rec(std::move(u_rec))
     {
     }
-    void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt, KV_double_3d const& rho, KV_double_4d const& u,
-                 KV_double_3d const& P, KV_double_4d const& fx, KV_double_5d const& rho_rec, KV_double_6d const& rhou_rec, KV_double_5d const& E_rec,
+    void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt, KV_cdouble_3d const& rho, KV_cdouble_4d const& u,
+                 KV_cdouble_3d const& P, KV_cdouble_4d const& fx, KV_double_5d const& rho_rec, KV_double_6d const& rhou_rec, KV_double_5d const& E_rec,
                  KV_double_6d const& fx_rec) const final
     {
         hclpp_range const r = range.cells();
-        check(hclpp_hydro_reconstruction(detail::ctx_for(grid, m_eos, gravity, detail::nfx_of(fx, 3)), m_face_reconstruction->limiter(), &r, dt,
-                                         rho.data(), u.data(), P.data(), fx.data(), rho_rec.data(), rhou_rec.data(), E_rec.data(), fx_rec.data(),
-                                         m_p_rec.data(), m_u_rec.data()));
+        int const nfx = detail::nfx_of(fx, 3);
+        auto const* face = dynamic_cast<LimitedLinearReconstruction const*>(m_face_reconstruction.get());
+        auto const* hancock = dynamic_cast<HancockExtrapolationReconstruction<Gravity> const*>(m_hancock_reconstruction.get());
+        if (face && hancock && hancock->eos().adiabatic_index() == m_eos.adiabatic_index()
+            && hancock->eos().mean_molecular_weight() == m_eos.mean_molecular_weight()) {
+            check(hclpp_hydro_reconstruction(detail::ctx_for(grid, m_eos, gravity, nfx), face->slope_limiter(), &r, dt, rho.data(), u.data(),
+                                             P.data(), fx.data(), rho_rec.data(), rhou_rec.data(), E_rec.data(), fx_rec.data(), m_p_rec.data(),
+                                             m_u_rec.data()));
+            return;
+        }
+        // hydro_reconstruction.hpp:99-109
+        m_face_reconstruction->execute(range, grid, rho, rho_rec);
+        for (int idim = 0; idim < grid.ndim; ++idim) m_face_reconstruction->execute(range, grid, u.slice(idim), m_u_rec.slice(idim));
+        m_face_reconstruction->execute(range, grid, P, m_p_rec);
+        for (int ifx = 0; ifx < nfx; ++ifx) m_face_reconstruction->execute(range, grid, fx.slice(ifx), fx_rec.slice(ifx));
+        // :111-126 (conv_prim_to_cons of every face state)
+        check(hclpp_face_prim_to_cons(detail::ctx_for(grid, m_eos), &r, rho_rec.data(), m_u_rec.data(), m_p_rec.data(), rhou_rec.data(),
+                                      E_rec.data()));
+        // :128
+        m_hancock_reconstruction->execute(range, grid, gravity, dt, m_u_rec, m_p_rec, rho_rec, rhou_rec, E_rec, fx_rec);
     }
 };
 
@@ -457,10 +523,10 @@ class IGodunovScheme
 {
 public:
     virtual ~IGodunovScheme() noexcept = default;
-    virtual void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt, KV_double_3d const& rho, KV_double_4d const& rhou,
-                         KV_double_3d const& E, KV_double_4d const& fx, KV_double_5d const& rho_rec, KV_double_6d const& rhou_rec,
-                         KV_double_5d const& E_rec, KV_double_6d const& fx_rec, KV_double_3d const& rho_new, KV_double_4d const& rhou_new,
-                         KV_double_3d const& E_new, KV_double_4d const& fx_new) const = 0;
+    virtual void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt, KV_cdouble_3d const& rho,
+                         KV_cdouble_4d const& rhou, KV_cdouble_3d const& E, KV_cdouble_4d const& fx, KV_cdouble_5d const& rho_rec,
+                         KV_cdouble_6d const& rhou_rec, KV_cdouble_5d const& E_rec, KV_cdouble_6d const& fx_rec, KV_double_3d const& rho_new,
+                         KV_double_4d const& rhou_new, KV_double_3d const& E_new, KV_double_4d const& fx_new) const = 0;
 };
 
 // :66-303
@@ -472,10 +538,10 @@ class RiemannBasedGodunovScheme : public IGodunovScheme<Gravity>
 
 public:
     RiemannBasedGodunovScheme(int solver, PerfectGas const& eos) : m_solver(solver), m_eos(eos) {}
-    void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt, KV_double_3d const& rho, KV_double_4d const& rhou,
-                 KV_double_3d const& E, KV_double_4d const& fx, KV_double_5d const& rho_rec, KV_double_6d const& rhou_rec, KV_double_5d const& E_rec,
-                 KV_double_6d const& fx_rec, KV_double_3d const& rho_new, KV_double_4d const& rhou_new, KV_double_3d const& E_new,
-                 KV_double_4d const& fx_new) const final
+    void execute(Range const& range, Grid const& grid, Gravity const& gravity, double dt, KV_cdouble_3d const& rho, KV_cdouble_4d const& rhou,
+                 KV_cdouble_3d const& E, KV_cdouble_4d const& fx, KV_cdouble_5d const& rho_rec, KV_cdouble_6d const& rhou_rec,
+                 KV_cdouble_5d const& E_rec, KV_cdouble_6d const& fx_rec, KV_double_3d const& rho_new, KV_double_4d const& rhou_new,
+                 KV_double_3d const& E_new, KV_double_4d const& fx_new) const final
     {
         hclpp_range const r = range.cells();
         check(hclpp_godunov(detail::ctx_for(grid, m_eos, gravity, detail::nfx_of(fx, 3)), m_solver, &r, dt, rho.data(), rhou.data(), E.data(),
@@ -494,8 +560,8 @@ std::unique_ptr<IGodunovScheme<Gravity>> factory_godunov_scheme(std::string cons
 }
 
 // ------------------------------------------------------------ hydro/time_step.hpp:35-82
-inline double time_step(Range const& range, PerfectGas const& eos, Grid const& grid, KV_double_3d const& rho, KV_double_4d const& u,
-                        KV_double_3d const& P)
+[[nodiscard]] inline double time_step(Range const& range, PerfectGas const& eos, Grid const& grid, KV_cdouble_3d const& rho,
+                                      KV_cdouble_4d const& u, KV_cdouble_3d const& P)
 {
     hclpp_range const r = range.cells();
     double dt = 0;
@@ -504,9 +570,9 @@ inline double time_step(Range const& range, PerfectGas const& eos, Grid const& g
 }
 
 // hydro/pressure_fix.hpp:19-172 (E_new updated in place)
-inline void pressure_fix(Range const& range, PerfectGas const& eos, Grid const& grid, double dt, double eps, KV_double_3d const& rho,
-                         KV_double_4d const& rhou, KV_double_3d const& E, KV_double_5d const& rho_rec, KV_double_6d const& rhou_rec,
-                         KV_double_5d const& E_rec, KV_double_3d const& rho_new, KV_double_4d const& rhou_new, KV_double_3d const& E_new)
+inline void pressure_fix(Range const& range, PerfectGas const& eos, Grid const& grid, double dt, double eps, KV_cdouble_3d const& rho,
+                         KV_cdouble_4d const& rhou, KV_cdouble_3d const& E, KV_cdouble_5d const& rho_rec, KV_cdouble_6d const& rhou_rec,
+                         KV_cdouble_5d const& E_rec, KV_cdouble_3d const& rho_new, KV_cdouble_4d const& rhou_new, KV_double_3d const& E_new)
 {
     hclpp_range const r = range.cells();
     check(hclpp_pressure_fix(detail::ctx_for(grid, eos), &r, dt, eps, rho.data(), rhou.data(), E.data(), rho_rec.data(), rhou_rec.data(),
@@ -514,21 +580,21 @@ inline void pressure_fix(Range const& range, PerfectGas const& eos, Grid const& 
 }
 
 // analysis/integration.cpp:22-37
-inline double integrate(Range const& range, Grid const& grid, KV_double_3d const& var)
+[[nodiscard]] inline double integrate(Range const& range, Grid const& grid, KV_cdouble_3d const& var)
 {
     hclpp_range const r = range.cells();
     double s = 0;
-    check(hclpp_integrate(detail::ctx_for(grid, PerfectGas(1.4, 1.0)), &r, var.data(), &s));
+    check(hclpp_integrate(grid.geometry_context(), &r, var.data(), &s));
     return s;
 }
 
 // analysis/internal_energy.cpp:20-45
-inline double minimum_internal_energy(Range const& range, Grid const& grid, KV_double_3d const& rho, KV_double_4d const& rhou,
-                                      KV_double_3d const& E)
+[[nodiscard]] inline double minimum_internal_energy(Range const& range, Grid const& grid, KV_cdouble_3d const& rho, KV_cdouble_4d const& rhou,
+                                                    KV_cdouble_3d const& E)
 {
     hclpp_range const r = range.cells();
     double m = 0;
-    check(hclpp_min_internal_energy(detail::ctx_for(grid, PerfectGas(1.4, 1.0)), &r, rho.data(), rhou.data(), E.data(), &m));
+    check(hclpp_min_internal_energy(grid.geometry_context(), &r, rho.data(), rhou.data(), E.data(), &m));
     return m;
 }
 
@@ -552,7 +618,7 @@ public:
     }
     void operator()(Grid const& grid, KV_double_3d const& rho, KV_double_4d const& rhou, KV_double_3d const& E, KV_double_4d const& fx) const
     {
-        hclpp_ctx* c = grid.context(PerfectGas(1.4, 1.0), HCLPP_GRAV_UNIFORM, {0, 0, 0}, 0.0, m_bc, m_order, detail::nfx_of(fx, 3));
+        hclpp_ctx* c = grid.geometry_context(m_bc, &m_order, detail::nfx_of(fx, 3));
         check(hclpp_fill_ghosts(c, rho.data(), rhou.data(), E.data(), fx.data()));
     }
     std::array<int, 6> const& kinds() const { return m_bc; }

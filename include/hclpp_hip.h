@@ -104,6 +104,13 @@ int hclpp_face_reconstruction(hclpp_ctx* ctx, int limiter, const hclpp_range* ra
  * hydro/extrapolation_reconstruction.hpp:27-54, :65-338.  dt is the half step. */
 int hclpp_hancock_extrapolation(hclpp_ctx* ctx, const hclpp_range* range, double dt, const double* u_rec, const double* P_rec,
                                 double* rho_rec, double* rhou_rec, double* E_rec, double* fx_rec);
+/* The face-state conversion of MUSCLHancockHydroReconstruction::execute,
+ * hydro/hydro_reconstruction.hpp:111-126: conv_prim_to_cons of every face state (iside, idim)
+ * over the range, from rho_rec, u_rec, P_rec (layouts [cell][side][dim] and
+ * [cell][side][dim][component]) to rhou_rec and E_rec.  For callers that compose the
+ * reconstruction from their own IFaceReconstruction / IExtrapolationReconstruction. */
+int hclpp_face_prim_to_cons(hclpp_ctx* ctx, const hclpp_range* range, const double* rho_rec, const double* u_rec, const double* P_rec,
+                            double* rhou_rec, double* E_rec);
 /* IHydroReconstruction::execute of MUSCLHancockHydroReconstruction (P_rec, u_rec are
  * its scratch views): hydro/hydro_reconstruction.hpp:30-59, :85-129 */
 int hclpp_hydro_reconstruction(hclpp_ctx* ctx, int limiter, const hclpp_range* range, double dt, const double* rho, const double* u,

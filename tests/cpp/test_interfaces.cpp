@@ -452,6 +452,142 @@ static void test_passive_scalar_loop()
     or_destroy(o);
 }
 
+// A user's own face reconstruction: the reference's limited linear reconstruction
+// (limited_linear_reconstruction.hpp:34-65) with the Minmod limiter, computed on the host.  Being
+// a user type, it makes MUSCLHancockHydroReconstruction compose the steps through the interfaces.
+class HostMinmodReconstruction : public IFaceReconstruction
+{
+public:
+    mutable int calls = 0;
+    void execute(Range const& range, Grid const& grid, KV_cdouble_3d const& var, KV_double_5d const& var_rec) const override
+    {
+        ++calls;
+        std::vector<double> const v = var.to_host();
+        std::vector<double> rec = var_rec.to_host();
+        auto const& w = grid.Nx_local_wg;
+        size_t const plane = static_cast<size_t>(w[0]) * w[1] * w[2];
+        hclpp_range const r = range.cells();
+        for (int k = r.begin[2]; k < r.end[2]; ++k)
+            for (int j = r.begin[1]; j < r.end[1]; ++j)
+                for (int i = r.begin[0]; i < r.end[0]; ++i) {
+                    int const idx[3] = {i, j, k};
+                    size_t const c = i + static_cast<size_t>(w[0]) * (j + static_cast<size_t>(w[1]) * k);
+                    for (int d = 0; d < grid.ndim; ++d) {
+                        size_t const st = d == 0 ? 1 : (d == 1 ? static_cast<size_t>(w[0]) : static_cast<size_t>(w[0]) * w[1]);
+                        auto const& x = grid.x[d];
+                        int const a = idx[d];
+                        double const dl = x[a + 1] - x[a], dlm = x[a] - x[a - 1], dlp = x[a + 2] - x[a + 1];
+                        double const slope = or_limiter(OR_LIM_MINMOD, (v[c + st] - v[c]) / ((dl + dlp) / 2), (v[c] - v[c - st]) / ((dlm + dl) / 2));
+                        rec[c + plane * (0 + 2 * d)] = v[c] - (dl / 2) * slope;
+                        rec[c + plane * (1 + 2 * d)] = v[c] + (dl / 2) * slope;
+                    }
+                }
+        var_rec.copy_from_host(rec);
+    }
+};
+
+// Another user type: forwards to the library's reconstruction and counts the calls.
+class CountingReconstruction : public IFaceReconstruction
+{
+    std::unique_ptr<IFaceReconstruction> m_inner;
+
+public:
+    mutable int calls = 0;
+    explicit CountingReconstruction(std::unique_ptr<IFaceReconstruction> inner) : m_inner(std::move(inner)) {}
+    void execute(Range const& range, Grid const& grid, KV_cdouble_3d const& var, KV_double_5d const& var_rec) const override
+    {
+        ++calls;
+        m_inner->execute(range, grid, var, var_rec);
+    }
+};
+
+// hydro_reconstruction.hpp:85-129 with user strategies: MUSCLHancockHydroReconstruction must call
+// the injected face reconstruction for rho, each velocity component, P and each passive scalar, and
+// the composed result must be bit-identical to the oracle's reconstruction (2D, Minmod, uniform
+// gravity, one passive scalar; the library's own strategies take the one-call path and must agree).
+static void test_user_face_reconstruction()
+{
+    int const nd = 2, n = 24, nfx = 1;
+    double const gamma = 1.4, dt = 3e-3;
+    Grid const grid(nd, {n, n - 5, 1}, 2, Geometry::Geom_cartesian, {0, 0, 0}, {1, 0.8, 1});
+    PerfectGas const eos(gamma, 1.);
+    UniformGravity const g({0.02, -0.1, 0.0});
+    auto const& w = grid.Nx_local_wg;
+    size_t const cells = static_cast<size_t>(w[0]) * w[1] * w[2];
+    std::vector<double> hrho(cells), hu(cells * nd), hP(cells), hfx(cells * nfx);
+    for (size_t c = 0; c < cells; ++c) {
+        hrho[c] = 1.0 + 0.4 * std::sin(0.37 * c) * std::cos(0.05 * c);
+        hu[c] = 0.3 * std::sin(0.11 * c + 1.0);
+        hu[c + cells] = -0.2 * std::cos(0.07 * c);
+        hP[c] = 1.0 + 0.3 * std::cos(0.23 * c);
+        hfx[c] = 0.5 + 0.5 * std::sin(0.19 * c);
+    }
+    or_config oc {};
+    oc.ndim = nd;
+    oc.geom = OR_GEOM_CARTESIAN;
+    oc.limiter = OR_LIM_MINMOD;
+    oc.riemann = OR_RS_HLLC;
+    oc.gravity = OR_GRAV_UNIFORM;
+    oc.nfx = nfx;
+    oc.ng = 2;
+    oc.n[0] = n;
+    oc.n[1] = n - 5;
+    oc.n[2] = 1;
+    for (int f = 0; f < 6; ++f) {
+        oc.bc[f] = OR_BC_PERIODIC;
+        oc.bc_order[f] = f;
+    }
+    oc.gamma = gamma;
+    oc.mmw = 1.;
+    oc.g[0] = 0.02;
+    oc.g[1] = -0.1;
+    for (int d = 0; d < 3; ++d) oc.x[d] = grid.x[d].data();
+    or_ctx* o = or_create(&oc);
+    int r6[6];
+    or_range(o, 1, r6);
+    size_t const fcells = cells * 2 * nd;
+    std::vector<double> orho(fcells, 0.), orhou(fcells * nd, 0.), oE(fcells, 0.), ofx(fcells * nfx, 0.), oP(fcells, 0.), ou(fcells * nd, 0.);
+    or_hydro_reconstruction(o, r6, dt, hrho.data(), hu.data(), hP.data(), hfx.data(), orho.data(), orhou.data(), oE.data(), ofx.data(),
+                            oP.data(), ou.data());
+    KV_double_3d rho("rho", w[0], w[1], w[2]), P("P", w[0], w[1], w[2]);
+    KV_double_4d u("u", w[0], w[1], w[2], nd), fx("fx", w[0], w[1], w[2], nfx);
+    rho.copy_from_host(hrho);
+    u.copy_from_host(hu);
+    P.copy_from_host(hP);
+    fx.copy_from_host(hfx);
+    for (int variant = 0; variant < 3; ++variant) {
+        KV_double_5d rho_rec("rho_rec", w[0], w[1], w[2], 2, nd), E_rec("E_rec", w[0], w[1], w[2], 2, nd), P_rec("P_rec", w[0], w[1], w[2], 2, nd);
+        KV_double_6d rhou_rec("rhou_rec", w[0], w[1], w[2], 2, nd, nd), u_rec("u_rec", w[0], w[1], w[2], 2, nd, nd),
+                fx_rec("fx_rec", w[0], w[1], w[2], 2, nd, nfx);
+        std::unique_ptr<IFaceReconstruction> face;
+        HostMinmodReconstruction* host = nullptr;
+        CountingReconstruction* counting = nullptr;
+        if (variant == 0) {
+            face = factory_face_reconstruction("Minmod");  // the library's own: one C-ABI call
+        } else if (variant == 1) {
+            auto c = std::make_unique<CountingReconstruction>(factory_face_reconstruction("Minmod"));
+            counting = c.get();
+            face = std::move(c);
+        } else {
+            auto h = std::make_unique<HostMinmodReconstruction>();
+            host = h.get();
+            face = std::move(h);
+        }
+        MUSCLHancockHydroReconstruction<UniformGravity> rec(std::move(face), std::make_unique<HancockExtrapolationReconstruction<UniformGravity>>(eos),
+                                                            eos, P_rec, u_rec);
+        rec.execute(grid.range.with_ghosts(1), grid, g, dt, rho, u, P, fx, rho_rec, rhou_rec, E_rec, fx_rec);
+        int const expected_calls = 1 + nd + 1 + nfx;
+        if (counting) EXPECT(counting->calls == expected_calls, "counting strategy called %d times", counting->calls);
+        if (host) EXPECT(host->calls == expected_calls, "host strategy called %d times", host->calls);
+        EXPECT(std::memcmp(rho_rec.to_host().data(), orho.data(), fcells * 8) == 0, "variant %d: rho_rec differs from the oracle", variant);
+        EXPECT(std::memcmp(rhou_rec.to_host().data(), orhou.data(), fcells * nd * 8) == 0, "variant %d: rhou_rec differs from the oracle", variant);
+        EXPECT(std::memcmp(E_rec.to_host().data(), oE.data(), fcells * 8) == 0, "variant %d: E_rec differs from the oracle", variant);
+        EXPECT(std::memcmp(fx_rec.to_host().data(), ofx.data(), fcells * nfx * 8) == 0, "variant %d: fx_rec differs from the oracle", variant);
+        EXPECT(std::memcmp(P_rec.to_host().data(), oP.data(), fcells * 8) == 0, "variant %d: P_rec differs from the oracle", variant);
+    }
+    or_destroy(o);
+}
+
 int main()
 {
     try {
@@ -461,6 +597,7 @@ int main()
         test_internal_mass_loop();
         test_pressure_fix_loop();
         test_passive_scalar_loop();
+        test_user_face_reconstruction();
     } catch (std::exception const& e) {
         std::printf("FAIL: exception %s\n", e.what());
         return 2;

@@ -118,10 +118,36 @@ def test_factories_reject_unknown_names():
         H.factory_godunov_scheme("Roe", H.PerfectGas(1.4, 1.0))
 
 
+class _CountingFace(H.IFaceReconstruction):
+    """A user's face reconstruction (forwards to the library's and counts the calls)."""
+
+    def __init__(self, inner):
+        self.inner, self.calls = inner, 0
+
+    def execute(self, range_, grid, var, var_rec):
+        self.calls += 1
+        self.inner.execute(range_, grid, var, var_rec)
+
+
+class _CountingHancock(H.IExtrapolationReconstruction):
+    """A user's extrapolation step (forwards to the library's and counts the calls)."""
+
+    def __init__(self, inner):
+        self.inner, self.calls = inner, 0
+
+    def execute(self, *args):
+        self.calls += 1
+        self.inner.execute(*args)
+
+
+@pytest.mark.parametrize("strategies", ["library", "user"])
 @pytest.mark.parametrize("gravity", ["Uniform", "Point_mass"])
 @pytest.mark.parametrize("nfx", [0, 2])
 @pytest.mark.parametrize("nd,geom,n", GEOMS)
-def test_hydro_reconstruction(cuda, nd, geom, n, nfx, gravity):
+def test_hydro_reconstruction(cuda, nd, geom, n, nfx, gravity, strategies):
+    """MUSCLHancockHydroReconstruction with the library's strategies (one C-ABI call) or with user
+    strategies (composed through the interfaces, as hydro_reconstruction.hpp:85-129 does): both
+    bit-identical to the oracle."""
     kw = dict(limiter="VanLeer", nfx=nfx, gravity=gravity, g=(0.3, -0.2, 0.1), central_mass=2e10)
     g, o = _grids(nd, geom, n, **kw)
     rho, u, P, fx = _random_prim(o, 3, nfx)
@@ -135,10 +161,14 @@ def test_hydro_reconstruction(cuda, nd, geom, n, nfx, gravity):
     d = [torch.zeros(x.shape, dtype=torch.float64, device="cuda") for x in outs]
     if not nfx:
         d[3] = None
-    rec = H.MUSCLHancockHydroReconstruction(H.factory_face_reconstruction("VanLeer"),
-                                            H.HancockExtrapolationReconstruction(eos), eos, d[4], d[5])
+    face, hancock = H.factory_face_reconstruction("VanLeer"), H.HancockExtrapolationReconstruction(eos)
+    if strategies == "user":
+        face, hancock = _CountingFace(face), _CountingHancock(hancock)
+    rec = H.MUSCLHancockHydroReconstruction(face, hancock, eos, d[4], d[5])
     rec.execute(g.range.with_ghosts(1), g, grav, dt / 2, _t(rho), _t(u), _t(P), _t(fx) if nfx else None, d[0], d[1],
                 d[2], d[3])
+    if strategies == "user":
+        assert face.calls == 1 + nd + 1 + nfx and hancock.calls == 1
     for k, name in enumerate(["rho_rec", "rhou_rec", "E_rec", "fx_rec", "P_rec", "u_rec"]):
         if d[k] is None:
             continue
