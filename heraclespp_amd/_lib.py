@@ -19,6 +19,7 @@ HCLPP_ERR_ARG = 1
 HCLPP_ERR_HIP = 2
 HCLPP_ERR_STATE = 3
 HCLPP_ERR_UNSUPPORTED = 4
+HCLPP_ERR_NEGATIVE_DT = 5  # clock status: the next output time is not after the current time
 
 GEOM = {"Cartesian": 0, "Spherical": 1}
 LIMITER = {"Constant": 0, "VanLeer": 1, "Minmod": 2, "VanAlbada": 3}
@@ -39,6 +40,7 @@ EXPORTS = [
     "hclpp_cast_f64_to_f32", "hclpp_cast_f32_to_f64", "hclpp_step_dt", "hclpp_clock_record_dt", "hclpp_step_adjoint",
     "hclpp_steps_dt", "hclpp_steps_adjoint", "hclpp_pressure_fix", "hclpp_internal_mass_gravity", "hclpp_gravity_table",
     "hclpp_profile_begin", "hclpp_profile_end", "hclpp_fused_plan", "hclpp_time_step_noalloc",
+    "hclpp_clock_set_outputs", "hclpp_clock_resume", "hclpp_clock_resume_f32", "hclpp_temperature",
 ]
 
 
@@ -81,6 +83,10 @@ class Clock(ctypes.Structure):
         ("max_iter", ctypes.c_int),
         ("done", ctypes.c_int),
         ("status", ctypes.c_int),
+        ("iter_output_id", ctypes.c_int),
+        ("time_output_id", ctypes.c_int),
+        ("output_due", ctypes.c_int),
+        ("exiting", ctypes.c_int),
     ]
 
 
@@ -124,6 +130,10 @@ def lib() -> ctypes.CDLL:
         "hclpp_fill_ghosts": [vp, dp, dp, dp, dp],
         "hclpp_clock_start": [vp, d, ip, d, d, ip, dp, dp, dp],
         "hclpp_clock_read": [vp, ctypes.POINTER(Clock)],
+        "hclpp_clock_set_outputs": [vp, ip, d, d, ip, ip],
+        "hclpp_clock_resume": [vp, dp, dp, dp],
+        "hclpp_clock_resume_f32": [vp, dp, dp, dp],
+        "hclpp_temperature": [vp, rp, dp, dp, dp],
         "hclpp_step": [vp, dp, dp, dp, dp, dp, dp],
         "hclpp_step_fx": [vp, dp, dp, dp, dp, dp, dp, dp, dp],
         "hclpp_clock_start_f32": [vp, d, ip, d, d, ip, dp, dp, dp],

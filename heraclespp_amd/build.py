@@ -17,7 +17,11 @@ VALU chains, and on MI355X that schedule gives the FP64 march kernel 2 % (2D)
 to 7 % (3D) more throughput at the same occupancy.  Scheduling reorders
 instructions only, so results are unchanged.
 
-Usage: python -m heraclespp_amd.build [-j N] [--verbose]
+libhclpp_io.so (output and restart files in the reference's HDF5 layout,
+csrc/io_hdf5.cpp) is host code: g++ against the HDF5 C library found under
+HCLPP_HDF5_ROOT (default /opt/conda), with that library's directory as rpath.
+
+Usage: python -m heraclespp_amd.build [-j N] [--verbose] [--ieee | --all | --io]
 """
 
 from __future__ import annotations
@@ -35,6 +39,8 @@ INCLUDE = os.path.join(ROOT, "include")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhclpp_hip.so")
 LIB_IEEE = os.path.join(HERE, "libhclpp_hip_ieee.so")
+LIB_IO = os.path.join(HERE, "libhclpp_io.so")
+HDF5_ROOT = os.environ.get("HCLPP_HDF5_ROOT", "/opt/conda")
 
 SOURCES = ["ref_kernels.hip", "step_fused.hip", "step_fused_fx.hip", "adjoint.hip", "abi.cpp"]
 EXTRA = {src: ["-mllvm", "-amdgpu-sched-strategy=max-ilp"] for src in ("step_fused.hip", "step_fused_fx.hip")}
@@ -97,12 +103,30 @@ def build(jobs: int | None = None, verbose: bool = False, ieee: bool | None = No
     return lib
 
 
+def build_io(verbose: bool = False) -> str:
+    """libhclpp_io.so: g++ on csrc/io_hdf5.cpp against the HDF5 C library."""
+    inc, libdir = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        raise RuntimeError(f"hdf5.h not found under {inc} (set HCLPP_HDF5_ROOT)")
+    tmp = LIB_IO + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I", inc, "-I", INCLUDE, os.path.join(CSRC, "io_hdf5.cpp"),
+           "-o", tmp, "-L", libdir, "-lhdf5", f"-Wl,-rpath,{libdir}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on io_hdf5.cpp:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB_IO)
+    return LIB_IO
+
+
 def build_all(jobs: int | None = None, verbose: bool = False) -> list:
-    """Both libraries, compiled concurrently."""
+    """The default and IEEE-division libraries (compiled concurrently) and the I/O library."""
     jobs = jobs or (os.cpu_count() or 1)
-    with ThreadPoolExecutor(max_workers=2) as ex:
+    with ThreadPoolExecutor(max_workers=3) as ex:
         futs = [ex.submit(build, max(1, jobs // 2), verbose, flag) for flag in (None, True)]
-        return [f.result() for f in futs]
+        fio = ex.submit(build_io, verbose)
+        return [f.result() for f in futs] + [fio.result()]
 
 
 def main() -> None:
@@ -110,9 +134,12 @@ def main() -> None:
     ap.add_argument("-j", type=int, default=None)
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--ieee", action="store_true", help="build libhclpp_hip_ieee.so (IEEE division everywhere)")
-    ap.add_argument("--all", action="store_true", help="build both libraries")
+    ap.add_argument("--all", action="store_true", help="build every library")
+    ap.add_argument("--io", action="store_true", help="build libhclpp_io.so only")
     a = ap.parse_args()
-    if a.all:
+    if a.io:
+        print(build_io(a.verbose))
+    elif a.all:
         print("\n".join(build_all(a.j, a.verbose)))
     else:
         print(build(a.j, a.verbose, True if a.ieee else None))

@@ -57,6 +57,10 @@ struct hclpp_ctx {
     double* adj_prim = nullptr;     // reverse step: primitives (NV planes)
     double* adj_face = nullptr;     // reverse step: face slots and slope-adjoint shares
     double* adj_partials = nullptr; // reverse step: dgamma partials
+    // output schedule applied by the next clock start (hclpp_clock_set_outputs)
+    int out_iter_freq = 0;
+    double out_time_first = 0, out_time_freq = 0;
+    int out_iter_id = -1, out_time_id = -1;
     std::vector<hipEvent_t> prof_events;  // pairs (start, stop) per timed fused launch
     int prof_max = 0;
     int prof_count = 0;
@@ -541,6 +545,17 @@ int hclpp_time_step_noalloc(hclpp_ctx* c, const hclpp_range* r, const double* rh
     return HCLPP_OK;
 }
 
+int hclpp_temperature(hclpp_ctx* c, const hclpp_range* r, const double* rho, const double* P, double* T)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    int const rc = check_range(c, r, 0);
+    if (rc) return rc;
+    if (!rho || !P || !T) return fail(HCLPP_ERR_ARG, "null array");
+    launch_temperature(c->G, to_range(r), c->eos, rho, P, T, c->stream);
+    HCLPP_LAUNCHED("fill_temperature_array");
+    return HCLPP_OK;
+}
+
 int hclpp_min_internal_energy(hclpp_ctx* c, const hclpp_range* r, const double* rho, const double* rhou, const double* E, double* out)
 {
     if (!c || !out) return fail(HCLPP_ERR_ARG, "null argument");
@@ -680,6 +695,11 @@ static int clock_reset(hclpp_ctx* c, double t0, int iter0, double t_end, double 
     h->pub.cfl = cfl;
     h->pub.max_iter = max_iter;
     h->pub.min_eint = NAN;
+    h->pub.iter_output_id = c->out_iter_id;
+    h->pub.time_output_id = c->out_time_id;
+    h->iter_output_frequency = c->out_iter_freq;
+    h->time_first_output = c->out_time_first;
+    h->time_output_frequency = c->out_time_freq;
     HCLPP_HIP(hipMemcpyAsync(c->clock, h, sizeof(DevClock), hipMemcpyHostToDevice, c->stream));
     Range3 all;
     for (int d = 0; d < 3; ++d) {
@@ -687,6 +707,70 @@ static int clock_reset(hclpp_ctx* c, double t0, int iter0, double t_end, double 
         all.e[d] = c->G.nwg[d];
     }
     return ensure_partials(c, reduce_blocks(all));
+}
+
+int hclpp_clock_set_outputs(hclpp_ctx* c, int iter_output_frequency, double time_first_output, double time_output_frequency, int iter_output_id,
+                            int time_output_id)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (iter_output_frequency < 0 || !(time_output_frequency >= 0) || !std::isfinite(time_first_output) || iter_output_id < -1
+        || time_output_id < -1)
+        return fail(HCLPP_ERR_ARG, "invalid output schedule");
+    c->out_iter_freq = iter_output_frequency;
+    c->out_time_first = time_first_output;
+    c->out_time_freq = time_output_frequency;
+    c->out_iter_id = iter_output_id;
+    c->out_time_id = time_output_id;
+    return HCLPP_OK;
+}
+
+// Continue a paused clock (output_due) on the state it stopped at: t, iter, the output counters and
+// the schedule carry over, and dt is computed afresh from the state, as the reference's next loop
+// iteration does (main.cpp:355).
+static int clock_resume_reset(hclpp_ctx* c)
+{
+    DevClock* h = c->host_clock;
+    HCLPP_HIP(hipMemcpyAsync(h, c->clock, sizeof(DevClock), hipMemcpyDeviceToHost, c->stream));
+    HCLPP_HIP(hipStreamSynchronize(c->stream));
+    if (h->pub.status) return fail(HCLPP_ERR_STATE, "the clock stopped on an error");
+    if (h->pub.exiting) return fail(HCLPP_ERR_ARG, "the run has ended (t_end or max_iter reached)");
+    h->pub.done = 0;
+    h->pub.output_due = 0;
+    h->pending_exit = 0;
+    h->pending_output = 0;
+    HCLPP_HIP(hipMemcpyAsync(c->clock, h, sizeof(DevClock), hipMemcpyHostToDevice, c->stream));
+    return HCLPP_OK;
+}
+
+int hclpp_clock_resume(hclpp_ctx* c, const double* rho, const double* rhou, const double* E)
+{
+    if (!c || !rho || !rhou || !E) return fail(HCLPP_ERR_ARG, "null argument");
+    if (c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_ARG, "the context was created for FP32 storage");
+    int rc = clock_resume_reset(c);
+    if (rc) return rc;
+    Range3 all;
+    for (int d = 0; d < 3; ++d) {
+        all.b[d] = 0;
+        all.e[d] = c->G.nwg[d];
+    }
+    launch_time_step_cons(c->cfg.ndim, c->cfg.geom, c->G, all, c->eos, rho, rhou, E, c->partials, c->scalars, c->stream);
+    launch_clock_start(c->clock, c->scalars, c->stream);
+    HCLPP_LAUNCHED("clock resume");
+    HCLPP_HIP(hipStreamSynchronize(c->stream));
+    return HCLPP_OK;
+}
+
+int hclpp_clock_resume_f32(hclpp_ctx* c, const float* rho, const float* rhou, const float* E)
+{
+    if (!c || !rho || !rhou || !E) return fail(HCLPP_ERR_ARG, "null argument");
+    if (c->cfg.precision != HCLPP_FP32_STORAGE) return fail(HCLPP_ERR_ARG, "the context was not created for FP32 storage");
+    int rc = clock_resume_reset(c);
+    if (rc) return rc;
+    launch_state_rate_f32(c->cfg.ndim, c->cfg.geom, c->G, c->eos32, rho, rhou, E, c->partials, c->scalars, c->stream);
+    launch_clock_start(c->clock, c->scalars, c->stream);
+    HCLPP_LAUNCHED("clock resume (fp32)");
+    HCLPP_HIP(hipStreamSynchronize(c->stream));
+    return HCLPP_OK;
 }
 
 int hclpp_clock_start_f32(hclpp_ctx* c, double t0, int iter0, double t_end, double cfl, int max_iter, const float* rho,

@@ -15,8 +15,13 @@ namespace hclpp {
 struct DevClock {
     hclpp_clock pub;
     int pending_exit;
+    int pending_output;  // the scheduled step ends at an output
     int dt_log_cap;   // entries of dt_log
     double* dt_log;   // when set, step `iter` records its dt at dt_log[iter] (iter < dt_log_cap)
+    // output schedule (main.cpp:362-375); frequencies of 0 disable it
+    int iter_output_frequency;
+    double time_first_output;
+    double time_output_frequency;
 };
 
 // Per-workgroup partials of the fused step: [0] max CFL rate, [1] min internal

@@ -99,6 +99,8 @@ void launch_time_step_rate(int nd, int geom, const GridDev& G, const Range3& r, 
 void launch_min_internal_energy(int nd, const GridDev& G, const Range3& r, const double* rho, const double* rhou, const double* E,
                                 double* partials, double* out, hipStream_t s);
 void launch_integrate(const GridDev& G, int nd, int geom, const Range3& r, const double* var, double* partials, double* out, hipStream_t s);
+// analysis/temperature.hpp:18-26
+void launch_temperature(const GridDev& G, const Range3& r, const PerfectGas& eos, const double* rho, const double* P, double* T, hipStream_t s);
 // Fold n partials (max) into out[0] in a fixed order.
 void launch_fold_max(const double* partials, int64_t n, double* out, hipStream_t s);
 // launch_time_step_rate folding to the reciprocal of the maximum (time_step.hpp:81) in device memory

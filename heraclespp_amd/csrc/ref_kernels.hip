@@ -553,6 +553,22 @@ void launch_time_step_cons(int nd, int geom, const GridDev& G, const Range3& r, 
     hipLaunchKernelGGL(k_fold<kMax>, dim3(1), dim3(1024), 0, s, partials, reduce_blocks(r), out);
 }
 
+// analysis/temperature.hpp:18-26 with perfect_gas.hpp:69-72 (IEEE division, as the reference)
+__global__ __launch_bounds__(256) void k_temperature(GridDev G, Range3 r, PerfectGas eos, const double* __restrict__ rho,
+                                                     const double* __restrict__ P, double* __restrict__ T)
+{
+    int i, j, k;
+    if (!range_cell(r, i, j, k)) return;
+    int64_t const id = flat(G, i, j, k);
+    T[id] = (P[id] * eos.mmw * kMp) / (rho[id] * kKb);
+}
+
+void launch_temperature(const GridDev& G, const Range3& r, const PerfectGas& eos, const double* rho, const double* P, double* T, hipStream_t s)
+{
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_temperature, range_grid(r), dim3(kBX, kBY), 0, s, G, r, eos, rho, P, T);
+}
+
 // k_fold<kMax> writing the reciprocal of the maximum: time_step.hpp:81 returns 1 / max
 __global__ __launch_bounds__(1024) void k_fold_max_inverse(const double* __restrict__ partials, int64_t n, double* __restrict__ out)
 {

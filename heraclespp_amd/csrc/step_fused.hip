@@ -103,18 +103,42 @@ __global__ __launch_bounds__(256) void k_cast(const In* __restrict__ in, Out* __
 }
 
 // ------------------------------------------------------------ clock
-// main.cpp:355-385 (dt, t_end clip, max_iter) and :397-425 (internal energy check)
+// main.cpp:355-385 (dt, output clipping, t_end clip, max_iter) and :397-425 (internal energy check)
 __device__ __forceinline__ void schedule_next(DevClock* c, double inv)
 {
     double dt = c->pub.cfl * (1 / inv);
-    int pending = 0;
+    int pending = 0, output = 0;
+    if (c->iter_output_frequency > 0) {  // :362-368 (iter_ini = 0)
+        int const next_output = (c->pub.iter_output_id + 1) * c->iter_output_frequency;
+        if ((c->pub.iter + 1) >= next_output) {
+            output = 1;
+            c->pub.iter_output_id += 1;
+        }
+    }
+    if (c->time_output_frequency > 0) {  // :369-375
+        double const next_output = c->time_first_output + ((c->pub.time_output_id + 1) * c->time_output_frequency);
+        if ((c->pub.t + dt) >= next_output) {
+            dt = next_output - c->pub.t;
+            if (dt <= 0) {
+                c->pub.status = HCLPP_ERR_NEGATIVE_DT;
+                return;
+            }
+            output = 1;
+            c->pub.time_output_id += 1;
+        }
+    }
     if ((c->pub.t + dt) >= c->pub.t_end) {
         dt = c->pub.t_end - c->pub.t;
+        output = 1;
         pending = 1;
     }
-    if ((c->pub.iter + 1) >= c->pub.max_iter) pending = 1;
+    if ((c->pub.iter + 1) >= c->pub.max_iter) {
+        output = 1;
+        pending = 1;
+    }
     c->pub.dt = dt;
     c->pending_exit = pending;
+    c->pending_output = output;
 }
 
 __global__ __launch_bounds__(64) void k_clock_start(DevClock* c, const double* inv)
@@ -129,6 +153,7 @@ __global__ __launch_bounds__(64) void k_clock_set_dt(DevClock* c, double dt)
     if ((c->pub.iter + 1) >= c->pub.max_iter) pending = 1;
     c->pub.dt = dt;
     c->pending_exit = pending;
+    c->pending_output = 0;
 }
 __global__ __launch_bounds__(64) void k_clock_set_log(DevClock* c, double* log, int cap)
 {
@@ -204,7 +229,13 @@ __global__ __launch_bounds__(1024) void k_clock_advance(DevClock* c, const doubl
     c->pub.mass = s[0];
     for (int d = 0; d < 3; ++d) c->pub.momentum[d] = d < NV - 2 ? s[1 + d] : 0.;
     c->pub.energy = s[NV - 1];
+    c->pub.output_due = c->pending_output;
     if (c->pending_exit) {
+        c->pub.done = 1;
+        c->pub.exiting = 1;
+        return;
+    }
+    if (c->pending_output) {  // pause: the caller writes the output, then starts the clock again
         c->pub.done = 1;
         return;
     }

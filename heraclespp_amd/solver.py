@@ -11,8 +11,10 @@ Two paths, both HIP:
     pressure fix, boundaries, cons->prim), one kernel per reference kernel,
     host-synchronised on dt exactly like the reference.  Needed for more than
     one passive scalar and for the pressure fix, and used to bisect parity.
-Time outputs are not modelled (the reference clips dt to output times only
-when an output frequency is configured).
+With an output schedule (simulate(), io.OutputSchedule) dt is clipped to the
+output times and outputs are written in the reference's HDF5 layout, as
+main.cpp:313-345 and :357-375, :532-557 do; restart() reads such a file back
+(main.cpp:223-238, then :305-311).
 """
 
 from __future__ import annotations
@@ -71,6 +73,10 @@ class HydroSolver:
         self.t = 0.0
         self.iter = 0
         self._clock_started = False
+        # output schedule of main.cpp:362-375 (frequencies 0: none) and the output counters
+        self._sched = (0, 0.0, 0.0)
+        self.output_id = self.iter_output_id = self.time_output_id = -1
+        self.restarted = False
         if not self.fused:
             self.rho_rec = z(nd, 2)
             self.rhou_rec = z(nd, nd, 2)
@@ -112,7 +118,7 @@ class HydroSolver:
                                             c._chk(self.u, (self.grid.ndim,)), c._chk(self.P)))
 
     # ---- main.cpp:305-311
-    def init_from_prim(self, rho, u, P, fx=None, t0: float = 0.0):
+    def init_from_prim(self, rho, u, P, fx=None, t0: float = 0.0, iter0: int = 0):
         g = self.grid
         nd = g.ndim
         rho_d = _dev(rho, g.shape3)
@@ -131,8 +137,101 @@ class HydroSolver:
         self._refresh_gravity()  # main.cpp:309
         self.update_primitives()
         self.t = float(t0)
-        self.iter = 0
+        self.iter = int(iter0)
         self._clock_started = False
+
+    # ---- main.cpp:223-238 (read_pdi), then :305-311
+    def restart(self, filename: str):
+        """Continue from an output file: read t, iter, the output counters and the primitive
+        state (rho, u, P and the passive scalars) of its interior, then initialise as
+        main.cpp:305-311 does (prim -> cons, boundaries, cons -> prim): E is rebuilt from P."""
+        from .io import read_state
+
+        hdr, rho, u, P, fx, _ = read_state(self.grid, filename, self.nfx)
+        self.init_from_prim(rho, u, P, fx, t0=hdr.current_time, iter0=hdr.iter)
+        self.output_id, self.iter_output_id, self.time_output_id = hdr.output_id, hdr.iter_output_id, hdr.time_output_id
+        self.restarted = True
+        return hdr
+
+    def set_outputs(self, iter_output_frequency: int = 0, time_first_output: float = 0.0, time_output_frequency: float = 0.0):
+        """Output schedule of main.cpp:362-375 for the following steps (frequencies 0: none)."""
+        self._sched = (int(iter_output_frequency), float(time_first_output), float(time_output_frequency))
+
+    def write_output(self, schedule) -> str:
+        """write_pdi (io.cpp:185-324) of the current state with the current output counters."""
+        from .io import Header, write_state
+
+        hdr = Header(self.output_id, self.iter_output_id, self.time_output_id, self.iter, self.t, self.eos.adiabatic_index())
+        return write_state(self.grid, self.eos, schedule.directory, schedule.prefix, hdr, self.rho, self.u, self.P, self.E,
+                           self.fx if self.nfx else None)
+
+    def simulate(self, t_end: float, max_iter: int, schedule=None, t_ini: float = 0.0) -> list:
+        """The time loop of main.cpp:313-557 with outputs: an initial output when a frequency is set and
+        the run is not a restart, dt clipped to the output times, an output after every step that ends
+        at one (the final step always does), and an output before the invalid-internal-energy error.
+        Returns the paths written."""
+        from .io import OutputSchedule
+
+        schedule = schedule or OutputSchedule()
+        written = []
+        if not self.restarted and schedule.enabled:  # main.cpp:313-345
+            self.iter_output_id += 1
+            self.output_id += 1
+            written.append(self.write_output(schedule))
+        self.set_outputs(schedule.iter_frequency, schedule.first_output(t_ini), schedule.time_frequency)
+        if self.fused:
+            self._simulate_fused(t_end, max_iter, schedule, written)
+        else:
+            while True:
+                try:
+                    done, make_output = self._step_operators(t_end, max_iter, outputs=True)
+                except _InvalidInternalEnergy:
+                    self.output_id += 1  # main.cpp:399-425: output, then the error
+                    written.append(self.write_output(schedule))
+                    raise
+                if make_output:
+                    self.output_id += 1
+                    written.append(self.write_output(schedule))
+                if done:
+                    break
+        return written
+
+    def _simulate_fused(self, t_end, max_iter, schedule, written, check_every: int = 64):
+        c = self.ctx
+        it, tf, ff = self._sched
+        check(lib().hclpp_clock_set_outputs(c.handle, it, tf, ff, self.iter_output_id, self.time_output_id))
+        self._clock_start(t_end, max_iter)
+        while True:
+            it0, cur0 = self.iter, self._cur
+            bufs = self._bufs()
+            launched = 0
+            while True:
+                for _ in range(check_every):
+                    self._launch_step(bufs[(cur0 + launched) % 2], bufs[(cur0 + launched + 1) % 2])
+                    launched += 1
+                clk = self.clock()
+                if clk.done or clk.status:
+                    break
+            steps = clk.iter - it0
+            self._cur = (cur0 + steps) % 2
+            self.t, self.iter = clk.t, clk.iter
+            self.iter_output_id, self.time_output_id = clk.iter_output_id, clk.time_output_id
+            self._sync_back()
+            self.update_primitives()
+            if clk.status == _lib.HCLPP_ERR_NEGATIVE_DT:
+                raise RuntimeError("Error: the time step is negative")
+            if clk.status:
+                self.output_id += 1  # main.cpp:399-425
+                written.append(self.write_output(schedule))
+                raise RuntimeError(f"Time = {clk.t}, iteration = {clk.iter}: detected invalid volumic internal energy")
+            if clk.output_due:
+                self.output_id += 1
+                written.append(self.write_output(schedule))
+            if clk.exiting:
+                return
+            r, m, E, _ = self._bufs()[self._cur]
+            resume = lib().hclpp_clock_resume_f32 if self.precision == "fp32" else lib().hclpp_clock_resume
+            check(resume(c.handle, _ptr(r), _ptr(m), _ptr(E)))
 
     # ---- main.cpp:354-531
     def run(self, t_end: float, max_iter: int, check_every: int = 64) -> int:
@@ -141,7 +240,7 @@ class HydroSolver:
             return self._run_fused(t_end, max_iter, check_every)
         n = 0
         while True:
-            done = self._step_operators(t_end, max_iter)
+            done, _ = self._step_operators(t_end, max_iter)
             n += 1
             if done:
                 return n
@@ -192,6 +291,7 @@ class HydroSolver:
     def _run_fused(self, t_end, max_iter, check_every):
         it0 = self.iter
         cur0 = self._cur
+        check(lib().hclpp_clock_set_outputs(self.ctx.handle, 0, 0.0, 0.0, -1, -1))  # no output pauses
         self._clock_start(t_end, max_iter)
         bufs = self._bufs()
         launched = 0
@@ -217,6 +317,7 @@ class HydroSolver:
 
     def start_clock(self, t_end: float, max_iter: int):
         """Start the device clock on the current state (for callers that drive step_async themselves)."""
+        check(lib().hclpp_clock_set_outputs(self.ctx.handle, 0, 0.0, 0.0, -1, -1))
         self._clock_start(t_end, max_iter)
         self._launched = 0
         self._cur0 = self._cur
@@ -250,7 +351,9 @@ class HydroSolver:
         if isinstance(self.gravity, InternalMassGravity):
             self.gravity.refresh(self.ctx, self.rho)
 
-    def _step_operators(self, t_end, max_iter) -> bool:
+    def _step_operators(self, t_end, max_iter, outputs: bool = False):
+        """One iteration of main.cpp:354-531; returns (exit, make_output).  With outputs, dt is
+        clipped to the schedule's output times and the output counters advance (:362-375)."""
         g = self.grid
         c = self.ctx
         nd = g.ndim
@@ -261,16 +364,31 @@ class HydroSolver:
                                 c._chk(self.P), ctypes.byref(inv)))
         dt = self.cfl * inv.value  # main.cpp:355
         exit_ = False
+        make_output = False
+        it_freq, t_first, t_freq = self._sched if outputs else (0, 0.0, 0.0)
+        if it_freq > 0 and (self.iter + 1) >= (self.iter_output_id + 1) * it_freq:  # :362-368
+            make_output = True
+            self.iter_output_id += 1
+        if t_freq > 0:  # :369-375
+            next_output = t_first + ((self.time_output_id + 1) * t_freq)
+            if (self.t + dt) >= next_output:
+                dt = next_output - self.t
+                if dt <= 0:
+                    raise RuntimeError("Error: the time step is negative")
+                make_output = True
+                self.time_output_id += 1
         if (self.t + dt) >= t_end:  # :377-381
             dt = t_end - self.t
             exit_ = True
+            make_output = True
         if (self.iter + 1) >= max_iter:  # :382-385
             exit_ = True
+            make_output = True
         me = ctypes.c_double()
         check(L.hclpp_min_internal_energy(h, ctypes.byref(self._r(0)), c._chk(self.rho), c._chk(self.rhou, (nd,)),
                                           c._chk(self.E), ctypes.byref(me)))
         if np.isnan(me.value) or me.value < 0:  # :397-425
-            raise RuntimeError(f"Time = {self.t}, iteration = {self.iter}: detected invalid volumic internal energy")
+            raise _InvalidInternalEnergy(f"Time = {self.t}, iteration = {self.iter}: detected invalid volumic internal energy")
         nfx = self.nfx
         fx = c._fx(self.fx, (nfx,), "fx")
         check(L.hclpp_hydro_reconstruction(h, _lib.LIMITER[c.limiter], ctypes.byref(self._r(1)), dt / 2,
@@ -296,7 +414,11 @@ class HydroSolver:
         self._refresh_gravity()  # :528
         self.t += dt
         self.iter += 1
-        return exit_
+        return exit_, make_output
+
+
+class _InvalidInternalEnergy(RuntimeError):
+    """main.cpp:420-424: the run stops on a NaN or negative internal energy."""
 
 
 def _ptr(t):

@@ -134,6 +134,9 @@ int hclpp_time_step(hclpp_ctx* ctx, const hclpp_range* range, const double* rho,
  * calls that overlap in time on one GPU (other streams or threads) race. */
 int hclpp_time_step_noalloc(hclpp_ctx* ctx, const hclpp_range* range, const double* rho, const double* u, const double* P,
                             double* dt_dev);
+/* temperature: analysis/temperature.hpp:18-26, T = compute_temp_from_pres(rho, P) over the range
+ * (perfect_gas.hpp:69-72, IEEE division). */
+int hclpp_temperature(hclpp_ctx* ctx, const hclpp_range* range, const double* rho, const double* P, double* T);
 /* minimum_internal_energy: analysis/internal_energy.cpp:20-45.  Synchronises. */
 int hclpp_min_internal_energy(hclpp_ctx* ctx, const hclpp_range* range, const double* rho, const double* rhou, const double* E,
                               double* out);
@@ -163,7 +166,11 @@ int hclpp_fill_ghosts(hclpp_ctx* ctx, double* rho, double* rhou, double* E, doub
  * t_end or max_iter is reached (main.cpp:355-385).  A step launched after
  * `done` is a no-op.  min_eint holds the minimum internal energy of the state
  * each step consumed (main.cpp:397-425); status becomes HCLPP_ERR_STATE when it
- * was NaN or negative, and every later step is a no-op. */
+ * was NaN or negative, and every later step is a no-op.
+ * With an output schedule (hclpp_clock_set_outputs), dt is also clipped to the next
+ * output time and output iterations are counted as main.cpp:362-375 does; the clock
+ * then pauses after every step that ends at an output (done = 1, output_due = 1, t and
+ * iter advanced): the caller writes the output and starts the clock again. */
 typedef struct {
     double t;        /* time of the current state */
     double dt;       /* step the next call will take */
@@ -176,8 +183,24 @@ typedef struct {
     int iter;
     int max_iter;
     int done;
-    int status;
+    int status;          /* 0, HCLPP_ERR_STATE (invalid internal energy) or HCLPP_ERR_NEGATIVE_DT */
+    int iter_output_id;  /* output counters of main.cpp:362-375 (-1 before the first output) */
+    int time_output_id;
+    int output_due;      /* the last step ended at an output (main.cpp:533-557: make_output) */
+    int exiting;         /* the last step was the final one (t_end or max_iter reached) */
 } hclpp_clock;
+
+/* status of a clock whose next output time is not after the current time
+ * (main.cpp:369-371: "Error: the time step is negative") */
+#define HCLPP_ERR_NEGATIVE_DT 5
+
+/* Output schedule of main.cpp:362-375 (utils/param.cpp:44-47), applied from the next
+ * hclpp_clock_start on: iteration outputs every iter_output_frequency steps (0: none), time
+ * outputs at time_first_output + k time_output_frequency (frequency 0: none), and the output
+ * counters to continue from (-1 for a new run; a restart resumes the file's counters).  The
+ * final step always ends at an output, as in the reference. */
+int hclpp_clock_set_outputs(hclpp_ctx* ctx, int iter_output_frequency, double time_first_output, double time_output_frequency,
+                            int iter_output_id, int time_output_id);
 
 /* Start the clock on a ghost-filled conservative state (FP64 arrays) at time t0 and
  * iteration iter0 (0 for a new run; a restart resumes the iteration count, main.cpp:223-238).
@@ -185,6 +208,12 @@ typedef struct {
  * iteration 0 as in main.cpp:382-385. */
 int hclpp_clock_start(hclpp_ctx* ctx, double t0, int iter0, double t_end, double cfl, int max_iter, const double* rho,
                       const double* rhou, const double* E);
+/* Continue a clock paused at an output (hclpp_clock.output_due) on the state it stopped at:
+ * t, iter, the output counters and the schedule carry over and dt is computed from the state,
+ * as the reference's next loop iteration does (main.cpp:355).  HCLPP_ERR_STATE after an error
+ * status, HCLPP_ERR_ARG once the run has ended. */
+int hclpp_clock_resume(hclpp_ctx* ctx, const double* rho, const double* rhou, const double* E);
+int hclpp_clock_resume_f32(hclpp_ctx* ctx, const float* rho, const float* rhou, const float* E);
 /* Copy the device clock to the host (synchronises). */
 int hclpp_clock_read(hclpp_ctx* ctx, hclpp_clock* out);
 /* One fused step: reads (rho, rhou, E) ghost-filled, writes the new state,

@@ -278,6 +278,9 @@ struct or_ctx {
     double *rho_new, *rhou_new, *E_new, *fx_new;
     double t;
     int iter;
+    /* output schedule, main.cpp:362-375 (utils/param.cpp:44-47) */
+    int iter_output_frequency, iter_output_id, time_output_id;
+    double time_first_output, time_output_frequency;
 };
 
 #define IDX(c, i, j, k) ((int64_t)(i) + (int64_t)(c)->nwg[0] * ((int64_t)(j) + (int64_t)(c)->nwg[1] * (int64_t)(k)))
@@ -301,6 +304,10 @@ or_ctx* or_create(const or_config* cfg)
     if (!isfinite(cfg->gamma) || cfg->gamma <= 1) { set_err("Invalid gamma"); return NULL; }
     if (!isfinite(cfg->mmw) || cfg->mmw <= 0) { set_err("Invalid mmw"); return NULL; }
     or_ctx* c = (or_ctx*)xcalloc(1, sizeof(or_ctx));
+    if (c) {
+        c->iter_output_id = -1; /* main.cpp:209-211 */
+        c->time_output_id = -1;
+    }
     c->cfg = *cfg;
     c->nd = cfg->ndim;
     c->nfx = cfg->nfx;
@@ -1014,7 +1021,28 @@ int or_init_from_prim(or_ctx* c, const double* rho, const double* u, const doubl
     return 0;
 }
 
+void or_set_outputs(or_ctx* c, int iter_output_frequency, double time_first_output, double time_output_frequency, int iter_output_id,
+                    int time_output_id)
+{
+    c->iter_output_frequency = iter_output_frequency;
+    c->time_first_output = time_first_output;
+    c->time_output_frequency = time_output_frequency;
+    c->iter_output_id = iter_output_id;
+    c->time_output_id = time_output_id;
+}
+
+void or_output_ids(const or_ctx* c, int* iter_output_id, int* time_output_id)
+{
+    *iter_output_id = c->iter_output_id;
+    *time_output_id = c->time_output_id;
+}
+
 int or_step(or_ctx* c, double cfl, double t_end, int max_iter, double* dt_out, int* exit_flag)
+{
+    return or_step_outputs(c, cfl, t_end, max_iter, dt_out, exit_flag, NULL);
+}
+
+int or_step_outputs(or_ctx* c, double cfl, double t_end, int max_iter, double* dt_out, int* exit_flag, int* output_flag)
 {
     int64_t const pl = c->plane;
     int const nd = c->nd;
@@ -1023,9 +1051,22 @@ int or_step(or_ctx* c, double cfl, double t_end, int max_iter, double* dt_out, i
     or_range(c, 1, r1);
     or_range(c, c->ng, rall);
     double dt = cfl * or_time_step(c, rall, c->rho, c->u, c->P); /* :355 */
-    int ex = 0;
-    if ((c->t + dt) >= t_end) { dt = t_end - c->t; ex = 1; }     /* :377-381 */
-    if ((c->iter + 1) >= max_iter) ex = 1;                          /* :382-385 */
+    int ex = 0, out = 0;
+    if (c->iter_output_frequency > 0) {                             /* :362-368 (iter_ini = 0) */
+        int const next_output = (c->iter_output_id + 1) * c->iter_output_frequency;
+        if ((c->iter + 1) >= next_output) { out = 1; ++c->iter_output_id; }
+    }
+    if (c->time_output_frequency > 0) {                             /* :369-375 */
+        double const next_output = c->time_first_output + ((c->time_output_id + 1) * c->time_output_frequency);
+        if ((c->t + dt) >= next_output) {
+            dt = next_output - c->t;
+            if (dt <= 0) { set_err("Error: the time step is negative"); return 2; }
+            out = 1;
+            ++c->time_output_id;
+        }
+    }
+    if ((c->t + dt) >= t_end) { dt = t_end - c->t; ex = 1; out = 1; } /* :377-381 */
+    if ((c->iter + 1) >= max_iter) { ex = 1; out = 1; }             /* :382-385 */
     double const me = or_min_internal_energy(c, r0, c->rho, c->rhou, c->E); /* :397-398 */
     if (isnan(me) || me < 0) { set_err("detected invalid volumic internal energy"); return 1; }
     or_hydro_reconstruction(c, r1, dt / 2, c->rho, c->u, c->P, c->fx, c->rho_rec, c->rhou_rec, c->E_rec, c->fx_rec,
@@ -1046,6 +1087,7 @@ int or_step(or_ctx* c, double cfl, double t_end, int max_iter, double* dt_out, i
     c->iter += 1;
     if (dt_out) *dt_out = dt;
     if (exit_flag) *exit_flag = ex;
+    if (output_flag) *output_flag = out;
     return 0;
 }
 

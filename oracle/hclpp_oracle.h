@@ -99,6 +99,12 @@ void or_range(const or_ctx* c, int nghost_eff, int* range6);
 int or_init_from_prim(or_ctx* c, const double* rho, const double* u, const double* P, const double* fx);
 /* one step; returns 0 ok, 1 invalid internal energy; *exit_flag set when t_end/max_iter reached */
 int or_step(or_ctx* c, double cfl, double t_end, int max_iter, double* dt_out, int* exit_flag);
+/* or_step with the output schedule of main.cpp:362-385: *output_flag = make_output; returns 2
+ * ("the time step is negative") when the next output time is not after t */
+int or_step_outputs(or_ctx* c, double cfl, double t_end, int max_iter, double* dt_out, int* exit_flag, int* output_flag);
+void or_set_outputs(or_ctx* c, int iter_output_frequency, double time_first_output, double time_output_frequency, int iter_output_id,
+                    int time_output_id);
+void or_output_ids(const or_ctx* c, int* iter_output_id, int* time_output_id);
 /* which: 0 rho,1 rhou,2 E,3 fx,4 u,5 P */
 double* or_state(or_ctx* c, int which);
 double or_time(const or_ctx* c);

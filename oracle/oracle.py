@@ -116,6 +116,10 @@ def lib():
         L.or_init_from_prim.argtypes = [vp, dp, dp, dp, dp]
         L.or_step.restype = ctypes.c_int
         L.or_step.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, ip]
+        L.or_step_outputs.restype = ctypes.c_int
+        L.or_step_outputs.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, ip, ip]
+        L.or_set_outputs.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.or_output_ids.argtypes = [vp, ip, ip]
         L.or_state.restype = dp
         L.or_state.argtypes = [vp, ctypes.c_int]
         L.or_time.restype = ctypes.c_double
@@ -335,6 +339,25 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(lib().or_last_error().decode())
         return dt.value, bool(ex.value)
+
+    def set_outputs(self, iter_output_frequency=0, time_first_output=0.0, time_output_frequency=0.0, iter_output_id=-1,
+                    time_output_id=-1):
+        """Output schedule of main.cpp:362-375 for the following steps."""
+        lib().or_set_outputs(self._h, int(iter_output_frequency), float(time_first_output), float(time_output_frequency),
+                             int(iter_output_id), int(time_output_id))
+
+    def step_outputs(self, cfl, t_end, max_iter=1 << 30):
+        """One step under the output schedule: (dt, exit, make_output)."""
+        dt, ex, out = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        rc = lib().or_step_outputs(self._h, cfl, t_end, max_iter, ctypes.byref(dt), ctypes.byref(ex), ctypes.byref(out))
+        if rc != 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        return dt.value, bool(ex.value), bool(out.value)
+
+    def output_ids(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        lib().or_output_ids(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def state(self, which):
         """Live view of the driver array: rho, rhou, E, fx, u, P."""

@@ -19,12 +19,13 @@ from oracle import oracle as orc
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    names = set()
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
-        src = open(h).read()
-        names |= set(re.findall(r"^\s*(?:int|const char\s*\*|void)\s+(hclpp_\w+)\s*\(", src, re.M))
-    return names
+def _declared(header="hclpp_hip.h"):
+    src = open(os.path.join(ROOT, "include", header)).read()
+    return set(re.findall(r"^\s*(?:int|const char\s*\*|void)\s+(hclpp_\w+)\s*\(", src, re.M))
+
+
+def test_every_header_is_checked():
+    assert {os.path.basename(h) for h in glob.glob(os.path.join(ROOT, "include", "*.h"))} == {"hclpp_hip.h", "hclpp_io.h"}
 
 
 def test_library_exports_every_declared_symbol():
@@ -34,6 +35,16 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in declared if not hasattr(L, n)]
     assert not missing, missing
     assert set(_lib.EXPORTS) == declared
+
+
+def test_io_library_exports_every_declared_symbol():
+    from heraclespp_amd import io as hio
+
+    L = hio.io_lib()
+    declared = _declared("hclpp_io.h")
+    assert len(declared) == 5
+    missing = [n for n in declared if not hasattr(L, n)]
+    assert not missing, missing
 
 
 def test_ctypes_config_matches_header_layout():
