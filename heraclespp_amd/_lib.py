@@ -41,6 +41,7 @@ EXPORTS = [
     "hclpp_steps_dt", "hclpp_steps_adjoint", "hclpp_pressure_fix", "hclpp_internal_mass_gravity", "hclpp_gravity_table",
     "hclpp_profile_begin", "hclpp_profile_end", "hclpp_fused_plan", "hclpp_time_step_noalloc",
     "hclpp_clock_set_outputs", "hclpp_clock_resume", "hclpp_clock_resume_f32", "hclpp_temperature",
+    "hclpp_set_pressure_fix",
 ]
 
 
@@ -131,6 +132,7 @@ def lib() -> ctypes.CDLL:
         "hclpp_clock_start": [vp, d, ip, d, d, ip, dp, dp, dp],
         "hclpp_clock_read": [vp, ctypes.POINTER(Clock)],
         "hclpp_clock_set_outputs": [vp, ip, d, d, ip, ip],
+        "hclpp_set_pressure_fix": [vp, ip, d],
         "hclpp_clock_resume": [vp, dp, dp, dp],
         "hclpp_clock_resume_f32": [vp, dp, dp, dp],
         "hclpp_temperature": [vp, rp, dp, dp, dp],

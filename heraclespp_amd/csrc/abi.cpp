@@ -25,6 +25,7 @@ namespace hclpp {
 // reference-shaped kernels reading conservative arrays (ref_kernels.hip)
 void launch_time_step_cons(int nd, int geom, const GridDev& G, const Range3& r, const PerfectGas& eos, const double* rho, const double* rhou,
                            const double* E, double* partials, double* out, hipStream_t s);
+int64_t reduce_blocks(const Range3& r);
 }  // namespace hclpp
 
 using namespace hclpp;
@@ -57,6 +58,11 @@ struct hclpp_ctx {
     double* adj_prim = nullptr;     // reverse step: primitives (NV planes)
     double* adj_face = nullptr;     // reverse step: face slots and slope-adjoint shares
     double* adj_partials = nullptr; // reverse step: dgamma partials
+    // pressure fix of the fused step (hclpp_set_pressure_fix)
+    int pf_on = 0;
+    double pf_eps = 1e-6;
+    double* fix_partials = nullptr;  // two doubles per workgroup of the interior range
+    int64_t fix_nb = 0;
     // output schedule applied by the next clock start (hclpp_clock_set_outputs)
     int out_iter_freq = 0;
     double out_time_first = 0, out_time_freq = 0;
@@ -123,6 +129,17 @@ double* noalloc_scratch(int dev)
     if (dev < 0 || dev >= kMaxDevices) return nullptr;
     std::lock_guard<std::mutex> lock(g_scratch_mutex);
     return g_scratch[dev];
+}
+
+// the interior cells of the grid (Range::no_ghosts)
+Range3 interior_range(const GridDev& G)
+{
+    Range3 r;
+    for (int d = 0; d < 3; ++d) {
+        r.b[d] = G.ngh[d];
+        r.e[d] = G.ngh[d] + G.n[d];
+    }
+    return r;
 }
 
 Range3 to_range(const hclpp_range* r)
@@ -366,6 +383,7 @@ int hclpp_ctx_destroy(hclpp_ctx* c)
     if (c->tables) (void)hipFree(c->tables);
     if (c->partials) (void)hipFree(c->partials);
     if (c->ghost_partials) (void)hipFree(c->ghost_partials);
+    if (c->fix_partials) (void)hipFree(c->fix_partials);
     if (c->scalars) (void)hipFree(c->scalars);
     if (c->host_scalars) (void)hipHostFree(c->host_scalars);
     if (c->clock) (void)hipFree(c->clock);
@@ -652,6 +670,8 @@ static int fused_step(hclpp_ctx* c, const S* rho, const S* rhou, const S* E, con
     A.fx = nfx > 0 ? fx : nullptr;
     A.fx_new = nfx > 0 ? fx_new : nullptr;
     A.nfx = nfx;
+    A.pf = f32 ? 0 : c->pf_on;
+    A.eps_pf = c->pf_eps;
     A.partials = c->partials;
     A.ghost_partials = c->ghost_partials;
     A.clock = c->clock;
@@ -673,9 +693,16 @@ static int fused_step(hclpp_ctx* c, const S* rho, const S* rhou, const S* E, con
         HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count + 1], c->stream));
         c->prof_count += 1;
     }
+    if constexpr (!f32) {
+        if (c->pf_on) {  // main.cpp:459-475: after the update, before the boundaries
+            launch_pressure_fix_fused(nd, c->cfg.geom, c->cfg.limiter, c->G, eos, A, interior_range(c->G), c->fix_partials, c->stream);
+            HCLPP_LAUNCHED("pressure fix (fused step)");
+        }
+    }
     launch_ghosts_rate(nd, c->cfg.geom, c->G, eos, c->bc, c->shell, A, c->stream);
     HCLPP_LAUNCHED("ghost fill");
-    launch_clock_advance(nd, c->clock, c->partials, c->fused_nb, c->ghost_partials, c->ghost_cap, c->stream);
+    launch_clock_advance(nd, c->clock, c->partials, c->fused_nb, c->ghost_partials, c->ghost_cap, c->pf_on && !f32 ? c->fix_partials : nullptr,
+                         c->pf_on && !f32 ? c->fix_nb : 0, c->stream);
     HCLPP_LAUNCHED("clock advance");
     return HCLPP_OK;
 }
@@ -707,6 +734,21 @@ static int clock_reset(hclpp_ctx* c, double t0, int iter0, double t_end, double 
         all.e[d] = c->G.nwg[d];
     }
     return ensure_partials(c, reduce_blocks(all));
+}
+
+int hclpp_set_pressure_fix(hclpp_ctx* c, int on, double eps)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (on && c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "the pressure fix runs with FP64 storage only");
+    if (on && !std::isfinite(eps)) return fail(HCLPP_ERR_ARG, "eps must be finite");
+    if (on && !c->fix_partials) {
+        int64_t const nb = reduce_blocks(interior_range(c->G));
+        HCLPP_HIP(hipMalloc(&c->fix_partials, sizeof(double) * 2 * (nb > 0 ? nb : 1)));
+        c->fix_nb = nb;
+    }
+    c->pf_on = on ? 1 : 0;
+    c->pf_eps = eps;
+    return HCLPP_OK;
 }
 
 int hclpp_clock_set_outputs(hclpp_ctx* c, int iter_output_frequency, double time_first_output, double time_output_frequency, int iter_output_id,
@@ -890,6 +932,7 @@ int step_adjoint(hclpp_ctx* c, double dt, const double* rho, const double* rhou,
     if (c->cfg.riemann != HCLPP_RS_HLL && c->cfg.riemann != HCLPP_RS_HLLC)
         return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step supports the HLL and HLLC solvers only");
     if (c->cfg.nfx > 0) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step has no passive scalars");
+    if (c->pf_on) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step does not differentiate the pressure fix");
     if (c->cfg.gravity == HCLPP_GRAV_INTERNAL_MASS)
         return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step does not differentiate the state-dependent internal-mass gravity");
     if (c->cfg.ng < 2 || c->cfg.ng > adj::kMaxGhosts) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step needs 2 <= ng <= 4");

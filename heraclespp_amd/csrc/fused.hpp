@@ -40,6 +40,10 @@ struct StepArraysT {
     const S* __restrict__ fx;  // passive scalars (mass fractions, [f][cell]); unused when nfx = 0
     S* __restrict__ fx_new;
     int nfx;
+    // pressure fix (FP64 only): cells whose state at time n has E - e_k < eps_pf e_k are left out
+    // of the march kernel's CFL maximum; k_pressure_fix_fused rewrites their energy and rate
+    int pf;
+    double eps_pf;
     double* __restrict__ partials;
     double* __restrict__ ghost_partials;
     DevClock* clock;
@@ -100,6 +104,9 @@ void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, c
 void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGasMixed& eos,
                           const StepArraysF& A, hipStream_t s);
 int march_occupancy_fx(int nd, int geom, int rs, int lim, bool fp32);
+// the fused step's pressure fix (pressure_fix_fused.hip): after k_step_march, over the interior r
+void launch_pressure_fix_fused(int nd, int geom, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, const Range3& r,
+                               double* fix_partials, hipStream_t s);
 GhostShell make_ghost_shell(const GridDev& G, int nd);
 int64_t ghost_blocks(const GhostShell& B);
 void launch_ghosts_rate(int nd, int geom, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const GhostShell& B, const StepArrays& A,
@@ -116,6 +123,7 @@ void launch_clock_start(DevClock* c, const double* inv, hipStream_t s);
 void launch_clock_set_dt(DevClock* c, double dt, hipStream_t s);
 void launch_clock_set_log(DevClock* c, double* log, int cap, hipStream_t s);
 void launch_clock_advance(int nd, DevClock* c, const double* partials, int64_t nblocks, const double* ghost_partials, int64_t nghost,
+                          const double* fix_partials, int64_t nfix,
                           hipStream_t s);
 
 }  // namespace hclpp

@@ -39,6 +39,7 @@
 #include "fused.hpp"
 #include "launch.hpp"
 #include "physics.hpp"
+#include "pressure_fix.hpp"
 #include "reduce.hpp"
 
 namespace hclpp {
@@ -470,6 +471,8 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
     T dtodv_prev = 0;
     State<ND, T> fsR_prev, Fm_prev, part_prev;  // in registers, or in s_park when kPark
     T old_rho_prev = 0, old_mom_prev = 0, src_prev = 0;
+    // pressure fix: the cell of plane h-1 is rewritten by k_pressure_fix_fused, which also supplies its rate
+    bool pf_prev = false, pf_cur = false;
     // passive scalars: mass fractions of planes h-1, h, h+1 and the prefetch; the ring cell's
     // prefetch; the right march face of plane h-1, its partial-density update and the
     // upwinded scalar flux of its left march face (godunov_scheme.hpp:263-290)
@@ -637,6 +640,9 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
                 // min internal energy of the consumed state (internal_energy.cpp:25-40), as its
                 // pressure recovery formed it
                 r_emin = reduce_combine<kMin>(r_emin, kCarry ? e0 : state_eint<ND>(eos, c0));
+                if constexpr (std::is_same_v<Eos, PerfectGas>) {
+                    if (A.pf) pf_cur = pressure_fix_applies(c0, A.eps_pf);  // pressure_fix.hpp:61-68
+                }
             }
             if constexpr (ND >= 2) {
                 State<ND, T> FLp[NXCH], FRp[NXCH];
@@ -783,8 +789,9 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
                         A.fx_new[id + G.plane * f] = y < T(0) ? T(0) : (y > T(1) ? T(1) : y);
                     }
                 }
-                // epilogue: CFL rate of the new state and the conserved totals (FP64)
-                r_inv = reduce_combine<kMax>(r_inv, state_rate<ND, GEOM>(G, eos, pi, pj, pk, un));
+                // epilogue: CFL rate of the new state and the conserved totals (FP64); a cell the
+                // pressure fix rewrites gets its rate from k_pressure_fix_fused
+                if (!pf_prev) r_inv = reduce_combine<kMax>(r_inv, state_rate<ND, GEOM>(G, eos, pi, pj, pk, un));
 #pragma unroll
                 for (int v = 0; v < NV; ++v) r_sum[v] += (double)un[v] * vol;
             }
@@ -822,6 +829,8 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
         old_rho_prev = old_rho;
         old_mom_prev = old_mom;
         src_prev = src_m;
+        pf_prev = pf_cur;
+        pf_cur = false;
         pm = p0;
         p0 = pp;
         c0 = cp;

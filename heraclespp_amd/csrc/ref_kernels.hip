@@ -8,6 +8,7 @@
 
 #include "launch.hpp"
 #include "physics.hpp"
+#include "pressure_fix.hpp"
 #include "reduce.hpp"
 
 namespace hclpp {
@@ -674,47 +675,8 @@ __global__ __launch_bounds__(256) void k_pressure_fix(GridDev G, Range3 r, Perfe
 #pragma unroll
     for (int d = 0; d < ND; ++d) cons_new[1 + d] = rhou_new[id + pl * d];
     cons_new[ND + 1] = E_new[id];
-    State<ND> const prim = cons_to_prim(cons, eos);
-    double const ekin = kinetic_cons(cons);
-    double const evol = E[id] - ekin;
-    double const evol_new = cons_new[ND + 1] - kinetic_cons(cons_new);
-    double const de_god = evol_new - evol;
-    if (!(evol < eps * ekin)) return;
-    double const eps2 = eps / 10;
-    double divU = 0, divUE = 0, einL = 0, einR = 0;
-    double alpha = (evol * evol) / (ekin * ekin * eps * eps);
-    if (evol < eps2 * ekin) alpha = 0;
-    double out = E[id];
-#pragma unroll
-    for (int idim = 0; idim < ND; ++idim) {
-        int const di = idim == 0, dj = idim == 1, dk = idim == 2;
-        int64_t const idm = flat(G, i - di, j - dj, k - dk);
-        int64_t const idp = flat(G, i + di, j + dj, k + dk);
-        State<ND> const primL = cons_to_prim(face(id, 0, idim), eos);
-        State<ND> const primR = cons_to_prim(face(id, 1, idim), eos);
-        State<ND> const prim_mR = cons_to_prim(face(idm, 1, idim), eos);
-        State<ND> const prim_pL = cons_to_prim(face(idp, 0, idim), eos);
-        State<ND> const cons_p = cell(idp);
-        State<ND> const cons_m = cell(idm);
-        double const dv = cell_volume<ND, GEOM>(G, i, j, k);
-#pragma unroll
-        for (int ipos = 0; ipos < ND; ++ipos) {
-            double const us = (prim_mR[1 + ipos] + primL[1 + ipos]) / 2;
-            double const us_p = (primR[1 + ipos] + prim_pL[1 + ipos]) / 2;
-            double const dsp = face_area<ND, GEOM>(G, ipos, i + di, j + dj, k + dk);
-            double const ds0 = face_area<ND, GEOM>(G, ipos, i, j, k);
-            divU += ((us_p * dsp) - (us * ds0)) / dv;
-            einR = (us_p > 0) ? E[id] - kinetic_cons(cons) : E[idp] - kinetic_cons(cons_p);
-            einL = (us < 0) ? E[id] - kinetic_cons(cons) : E[idm] - kinetic_cons(cons_m);
-            divUE += ((us_p * einR * dsp) - (us * einL * ds0)) / dv;
-            double const de_pf = dt * ((-prim[ND + 1] * divU) - divUE);
-            double sum = 0;
-#pragma unroll
-            for (int idr = 0; idr < ND; ++idr) sum += cons[1 + idr] * cons[1 + idr] / cons[0];
-            double const de_tot = (alpha * de_god) + ((1 - alpha) * de_pf);
-            out = evol + de_tot + (0.5 * sum);
-        }
-    }
+    if (!pressure_fix_applies(cons, eps)) return;
+    double const out = pressure_fix_energy<ND, GEOM>(G, eos, dt, eps, i, j, k, cons, cons_new, cell, face);
     E_new[id] = out;
 }
 

@@ -164,7 +164,8 @@ __global__ __launch_bounds__(64) void k_clock_set_log(DevClock* c, double* log, 
 
 template <int NV>
 __global__ __launch_bounds__(1024) void k_clock_advance(DevClock* c, const double* __restrict__ partials, int64_t nblocks,
-                                                        const double* __restrict__ ghost_partials, int64_t nghost)
+                                                        const double* __restrict__ ghost_partials, int64_t nghost,
+                                                        const double* __restrict__ fix_partials, int64_t nfix)
 {
     __shared__ double lds[16][NV + 2];  // per-wave results of every reduced value: one LDS round
     if (c->pub.done || c->pub.status) return;
@@ -193,6 +194,11 @@ __global__ __launch_bounds__(1024) void k_clock_advance(DevClock* c, const doubl
         }
     }
     for (int64_t t = threadIdx.x; t < nghost; t += 1024) inv = reduce_combine<kMax>(inv, ghost_partials[t]);
+    // pressure fix: rates of the rewritten cells and the change of their energy times dV
+    for (int64_t t = threadIdx.x; t < nfix; t += 1024) {
+        inv = reduce_combine<kMax>(inv, fix_partials[2 * t]);
+        s[NV - 1] += fix_partials[2 * t + 1];
+    }
     // all NV + 2 values reduced together: wave shuffles, one barrier, then thread 0 folds the
     // 16 wave results in wave order (a fixed order: the totals are deterministic)
     inv = wave_reduce<kMax>(inv);
@@ -401,12 +407,18 @@ void launch_clock_set_log(DevClock* c, double* log, int cap, hipStream_t s)
 }
 
 void launch_clock_advance(int nd, DevClock* c, const double* partials, int64_t nblocks, const double* ghost_partials, int64_t nghost,
-                          hipStream_t s)
+                          const double* fix_partials, int64_t nfix, hipStream_t s)
 {
     switch (nd) {
-    case 1: hipLaunchKernelGGL(k_clock_advance<3>, dim3(1), dim3(1024), 0, s, c, partials, nblocks, ghost_partials, nghost); break;
-    case 2: hipLaunchKernelGGL(k_clock_advance<4>, dim3(1), dim3(1024), 0, s, c, partials, nblocks, ghost_partials, nghost); break;
-    default: hipLaunchKernelGGL(k_clock_advance<5>, dim3(1), dim3(1024), 0, s, c, partials, nblocks, ghost_partials, nghost); break;
+    case 1:
+        hipLaunchKernelGGL(k_clock_advance<3>, dim3(1), dim3(1024), 0, s, c, partials, nblocks, ghost_partials, nghost, fix_partials, nfix);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_clock_advance<4>, dim3(1), dim3(1024), 0, s, c, partials, nblocks, ghost_partials, nghost, fix_partials, nfix);
+        break;
+    default:
+        hipLaunchKernelGGL(k_clock_advance<5>, dim3(1), dim3(1024), 0, s, c, partials, nblocks, ghost_partials, nghost, fix_partials, nfix);
+        break;
     }
 }
 

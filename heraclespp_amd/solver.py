@@ -4,13 +4,14 @@
 Two paths, both HIP:
   * fused (default): one hclpp_step (hclpp_step_fx with one passive scalar)
     per iteration -- the fused reconstruct/predict/Riemann/update kernel, the
-    ghost fill and the device clock (dt, t_end clipping, max_iter, the
-    internal-energy check) -- with no host synchronisation inside the loop;
+    pressure fix when it is on, the ghost fill and the device clock (dt, t_end
+    clipping, max_iter, the internal-energy check) -- with no host
+    synchronisation inside the loop;
   * operators: the reference operator sequence (time_step,
     minimum_internal_energy, MUSCL-Hancock reconstruction, Godunov update,
     pressure fix, boundaries, cons->prim), one kernel per reference kernel,
     host-synchronised on dt exactly like the reference.  Needed for more than
-    one passive scalar and for the pressure fix, and used to bisect parity.
+    one passive scalar, and used to bisect parity.
 With an output schedule (simulate(), io.OutputSchedule) dt is clipped to the
 output times and outputs are written in the reference's HDF5 layout, as
 main.cpp:313-345 and :357-375, :532-557 do; restart() reads such a file back
@@ -46,7 +47,7 @@ class HydroSolver:
             raise ValueError("precision must be 'fp64' or 'fp32' (FP32 storage with FP64 pressure recovery)")
         if pressure_fix not in ("On", "Off"):
             raise ValueError("pressure_fix must be 'On' or 'Off'")
-        self.pressure_fix = pressure_fix == "On"  # main.cpp:459 (operator path: it needs the face states)
+        self.pressure_fix = pressure_fix == "On"  # main.cpp:459
         self.eps_pf = float(eps_pf)
         if precision == "fp32" and (not fused or nfx > 1 or self.pressure_fix):
             raise ValueError("FP32 storage is a fused-step mode (at most one passive scalar, no pressure fix)")
@@ -56,9 +57,11 @@ class HydroSolver:
         self.cfl = float(cfl)
         self.nfx = nfx
         # the fused step carries at most one passive scalar; the operator path takes any nfx
-        self.fused = fused and nfx <= 1 and not self.pressure_fix
+        self.fused = fused and nfx <= 1
         self.precision = precision
         self.ctx = Context(grid, eos, gravity, bc, bc_order, nfx, reconstruction, riemann_solver, precision)
+        if self.fused and self.pressure_fix:  # the fused step recomputes the face states the fix reads
+            check(lib().hclpp_set_pressure_fix(self.ctx.handle, 1, self.eps_pf))
         nd = grid.ndim
         z = grid.zeros
         self._cons = [(z(), z(nd), z()), (z(), z(nd), z())]

@@ -194,6 +194,13 @@ typedef struct {
  * (main.cpp:369-371: "Error: the time step is negative") */
 #define HCLPP_ERR_NEGATIVE_DT 5
 
+/* The pressure fix of main.cpp:459-475 (hydro/pressure_fix.hpp:19-172, "Pressure fix" = On,
+ * eps_pf) in every following hclpp_step / hclpp_step_fx: after the update, the energy of each
+ * cell whose state at time n has E - e_k < eps e_k is rewritten from the step's face states, as
+ * hclpp_pressure_fix does, and the CFL rate and energy total take the rewritten energies.  FP64
+ * storage only; on = 0 turns it off.  The reverse step refuses a context with the fix on. */
+int hclpp_set_pressure_fix(hclpp_ctx* ctx, int on, double eps);
+
 /* Output schedule of main.cpp:362-375 (utils/param.cpp:44-47), applied from the next
  * hclpp_clock_start on: iteration outputs every iter_output_frequency steps (0: none), time
  * outputs at time_first_output + k time_output_frequency (frequency 0: none), and the output

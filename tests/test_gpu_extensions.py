@@ -134,31 +134,62 @@ def test_internal_mass_gravity_steps(cuda, nd, n, steps, fused):
         assert l1 <= 1e-12, (name, l1)
 
 
-@pytest.mark.parametrize("nd,n,steps", [(1, [64], 10), (2, [32, 24], 12)])
-def test_pressure_fix_steps(cuda, nd, n, steps):
-    """Cold uniform stream (Mach ~800, e / ek ~ 3e-6, so every cell is flagged and blended) with
-    the pressure fix on (operator path): bit-identical to the oracle.  The reference's fix adds
-    back the old kinetic energy (pressure_fix.hpp:156-160), so it stays positive only where the
-    kinetic energy barely changes: a uniform velocity and a nearly uniform density."""
-    bc = ["Periodic"] * (2 * nd)
-    o = orc.Oracle(nd, n, bc=bc, limiter="VanLeer", riemann="HLL", pressure_fix=True, eps_pf=1e-5)
+def _pf_run(nd, n, steps, *, geom="Cartesian", lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), mixed=False, fused=True, limiter="VanLeer"):
+    """Cold stream of uniform velocity (Mach ~800, e / ek ~ 3e-6: the pressure fix rewrites every
+    cell), or with mixed=True a warm half (P = 1) beside it, so only some cells are rewritten; with
+    the fix on for `steps` steps, GPU (fused or operator path) against the oracle, bit for bit, the
+    clock included."""
+    bc = (["Periodic"] * (2 * nd) if geom == "Cartesian"
+          else ["Reflexive", "NullGradient", "Reflexive", "Reflexive", "Periodic", "Periodic"][:2 * nd])
+    kw = dict(geom=geom, xmin=lo, xmax=hi, bc=bc, limiter=limiter, riemann="HLL")
+    o = orc.Oracle(nd, n, pressure_fix=True, eps_pf=1e-5, **kw)
     rng = np.random.default_rng(3)
     rho = 1.0 + 1e-6 * rng.random(o.shape3)
-    u = 3.0 * np.ones((nd,) + o.shape3)
+    u = np.zeros((nd,) + o.shape3)
+    if geom == "Cartesian":
+        u[...] = 3.0
+    else:
+        u[0] = 3.0  # a radial stream
     P = 1e-5 * (1.0 + 0.5 * rng.random(o.shape3))
+    if mixed:
+        P[..., : P.shape[-1] // 2] = 1.0
     o.init_from_prim(rho, u, P)
     for _ in range(steps):
         o.step(0.4, 1e30, steps)
-    ref = orc.Oracle(nd, n, bc=bc, limiter="VanLeer", riemann="HLL")
+    ref = orc.Oracle(nd, n, **kw)
     ref.init_from_prim(rho, u, P)
     for _ in range(steps):
         ref.step(0.4, 1e30, steps)
     assert np.sum(o.state("E") != ref.state("E")) > 0  # the fix acted
-    g = H.Grid(nd, n, 2)
-    sol = HydroSolver(g, H.PerfectGas(1.4, 1.0), reconstruction="VanLeer", riemann_solver="HLL", bc=bc, cfl=0.4,
-                      pressure_fix="On", eps_pf=1e-5)
-    assert not sol.fused
+    g = H.Grid(nd, n, 2, geom, lo, hi)
+    sol = HydroSolver(g, H.PerfectGas(1.4, 1.0), reconstruction=limiter, riemann_solver="HLL", bc=bc, cfl=0.4,
+                      pressure_fix="On", eps_pf=1e-5, fused=fused)
+    assert sol.fused == fused
     sol.init_from_prim(rho, u, P)
-    sol.run(1e30, steps)
+    assert sol.run(1e30, steps) == steps
+    assert sol.t == o.t
     for name, gpu in (("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E)):
         np.testing.assert_array_equal(_n(gpu), o.state(name), err_msg=name)
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "operators"])
+@pytest.mark.parametrize("nd,n,steps", [(1, [64], 10), (2, [32, 24], 12), (3, [12, 10, 9], 6)])
+def test_pressure_fix_steps(cuda, nd, n, steps, fused):
+    """The reference's fix adds back the old kinetic energy (pressure_fix.hpp:156-160), so it stays
+    positive only where the kinetic energy barely changes: a uniform velocity and a nearly uniform
+    density."""
+    _pf_run(nd, n, steps, fused=fused)
+
+
+@pytest.mark.parametrize("nd,n", [(1, [64]), (2, [32, 24]), (3, [12, 10, 9])])
+def test_pressure_fix_fused_rewrites_some_cells(cuda, nd, n):
+    """Half the cells rewritten: the fused step's CFL maximum mixes the march kernel's rates (cells the
+    fix leaves alone) and the fix kernel's (rewritten cells); the second step's dt, and so the state,
+    must still be the oracle's.  Two steps: the reference's fix turns the warm/cold contact
+    non-physical after that."""
+    _pf_run(nd, n, 2, mixed=True, limiter="Minmod")
+
+
+@pytest.mark.parametrize("nd,n", [(2, [16, 8]), (3, [12, 6, 8])])
+def test_pressure_fix_fused_spherical(cuda, nd, n):
+    _pf_run(nd, n, 10, geom="Spherical", lo=(0.5, 0.3, 0.0), hi=(1.5, np.pi - 0.3, 2 * np.pi))
