@@ -60,6 +60,32 @@ __device__ __forceinline__ void ijk_of(int a, int b, int m, int& i, int& j, int&
     else { i = a; j = b; k = m; }
 }
 
+// Register exchange with the neighbouring column along x: a tile row is a DPP row (3D: rows of
+// 16 lanes, row_shr / row_shl) or the whole wave (2D: 64 columns, wave_shr / wave_shl), so the
+// column at x - 1 (FROM_LEFT) or x + 1 is the lane beside this one.  Every lane of the wave must
+// execute the shift; the first (FROM_LEFT) or last lane of a row receives 0.
+template <int ND, bool FROM_LEFT>
+__device__ __forceinline__ int lane_shift_u32(int v)
+{
+    constexpr int kCtrl = ND == 2 ? (FROM_LEFT ? 0x138 : 0x130) : (FROM_LEFT ? 0x111 : 0x101);
+    return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, true);
+}
+
+template <int ND, bool FROM_LEFT>
+__device__ __forceinline__ double lane_shift(double v)
+{
+    int2 p = __builtin_bit_cast(int2, v);
+    p.x = lane_shift_u32<ND, FROM_LEFT>(p.x);
+    p.y = lane_shift_u32<ND, FROM_LEFT>(p.y);
+    return __builtin_bit_cast(double, p);
+}
+
+template <int ND, bool FROM_LEFT>
+__device__ __forceinline__ float lane_shift(float v)
+{
+    return __builtin_bit_cast(float, lane_shift_u32<ND, FROM_LEFT>(__builtin_bit_cast(int, v)));
+}
+
 template <int ND, class S>
 __device__ __forceinline__ State<ND, S> load_cons(const StepArraysT<S>& A, const GridDev& G, int64_t id)
 {
@@ -302,9 +328,9 @@ __device__ __forceinline__ State<ND, T> dummy_state()
 
 // Minimum waves per SIMD the register allocator must allow.  Measured on MI355X: the 3D
 // FP32-storage kernel gains 23 % at 3 waves (168 VGPRs); the 3D Cartesian FP64 kernel gains 30 %
-// at 2 waves (256 VGPRs, 44 B of spills per lane, with the LDS parking and the shared exchange
-// buffer below so that two workgroups fit a CU's LDS); the spherical 3D FP64 kernel loses 14 %
-// at 2 waves (224 B of spills) and the 2D kernels lose when forced above their natural 2 or 4.
+// at 2 waves (256 VGPRs, 44 B of spills per lane, with the LDS parking below; two workgroups
+// fit a CU's LDS); the spherical 3D FP64 kernel loses 14 % at 2 waves (224 B of spills) and the
+// 2D kernels lose when forced above their natural 2 or 4.
 template <class Eos, int ND, int GEOM>
 constexpr int march_min_waves()
 {
@@ -338,16 +364,14 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
     constexpr int TX = (ND >= 2) ? HX - 2 : 1;
     constexpr int TY = (ND == 3) ? HY - 2 : 1;
     constexpr int NXCH = NP > 0 ? NP : 1;
-    // Three barriers per plane: the primitive plane is double-buffered by plane parity, and
-    // face states and fluxes have buffers of their own, so no write waits for the readers of
-    // the previous plane's buffer beyond the barrier that already separates them.
+    // Barriers per plane: one after the primitive plane (double-buffered by plane parity) and, in
+    // 3D, one after the y face states and one after the y fluxes, which have buffers of their own,
+    // so no write waits for the readers of the previous plane's buffer beyond the barrier that
+    // already separates them.  x faces are exchanged between lanes (lane_shift), not through LDS.
     __shared__ T s_prim[2][NL][PY][PX];
-    // 3D Cartesian FP64 (two workgroups per CU): the fluxes reuse the face-state buffer, at the
-    // cost of one more barrier per plane, so that the two workgroups fit a CU's LDS.
-    constexpr bool kShare = std::is_same_v<Eos, PerfectGas> && ND == 3 && GEOM == kCartesian;
-    __shared__ T s_st[NXCH][NL][HY][HX];
-    __shared__ T s_fl_own[kShare ? 1 : NXCH][NL][HY][HX];
-    T (*s_fl)[NL][HY][HX] = kShare ? s_st : s_fl_own;
+    constexpr int NYB = ND == 3 ? 1 : 0;  // LDS exchange buffers (the y dim of a 3D tile)
+    __shared__ T s_st[NYB > 0 ? NYB : 1][NYB > 0 ? NL : 1][NYB > 0 ? HY : 1][NYB > 0 ? HX : 1];
+    __shared__ T s_fl[NYB > 0 ? NYB : 1][NYB > 0 ? NL : 1][NYB > 0 ? HY : 1][NYB > 0 ? HX : 1];
     __shared__ double s_red[NT / 64];
     // Per-thread parking of the values plane h-1 hands to plane h (its right march face state,
     // its partial update and its left march flux, and the scalar's): written at the end of a
@@ -645,85 +669,88 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
                 }
             }
             if constexpr (ND >= 2) {
-                State<ND, T> FLp[NXCH], FRp[NXCH];
-                X XLp[NXCH], XRp[NXCH];  // upwinded scalar fluxes of the left and right faces
-                // right face states of every in-plane dim; s_st was last read before the
-                // flux barrier of the previous plane
-                if (in_block) {
-#pragma unroll
-                    for (int p = 0; p < NP; ++p) {
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) s_st[p][v][hy][hx] = fs[1][p][v];
-#pragma unroll
-                        for (int f = 0; f < NFX; ++f) s_st[p][NV + f][hy][hx] = xf[1][p].f[f];
-                    }
-                }
-                __syncthreads();
-#pragma unroll
-                for (int p = 0; p < NP; ++p) {
-                    bool need;
-                    if (p == 0) need = hancock_ok && hx >= 1 && (ND == 2 || (hy >= 1 && hy <= TY));
-                    else need = hancock_ok && hy >= 1 && hx >= 1 && hx <= TX;
-                    State<ND, T> F;
-                    X XF = zero_frac<NFX, T>();
+                // Riemann solve of the left face along in-plane dim p, given the right face state
+                // (and scalar face values) of the column on the left; a column that needs no
+                // such face yields its own state as a placeholder flux
+                auto solve = [&](int p, bool need, const State<ND, T>& left, const X& xleft, State<ND, T>& F, X& XF) {
+                    XF = zero_frac<NFX, T>();
                     if (need) {
-                        State<ND, T> left;
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) left[v] = (p == 0) ? s_st[p][v][hy][hx - 1] : s_st[p][v][hy - 1][hx];
                         F = riemann<RS>(left, fs[0][p], p, eos);
                         // upwind face of the scalar (godunov_scheme.hpp:265-287): this cell's
                         // left face when the mass flux is negative, else the left cell's right face
 #pragma unroll
-                        for (int f = 0; f < NFX; ++f) {
-                            T const up = F[0] < 0 ? xf[0][p].f[f] : ((p == 0) ? s_st[p][NV + f][hy][hx - 1] : s_st[p][NV + f][hy - 1][hx]);
-                            XF.f[f] = up * F[0];
-                        }
+                        for (int f = 0; f < NFX; ++f) XF.f[f] = (F[0] < 0 ? xf[0][p].f[f] : xleft.f[f]) * F[0];
                     } else {
                         F = c0;
                     }
-                    FLp[p] = F;
-                    XLp[p] = XF;
-                    // s_fl was last read before the primitive barrier of this plane
-                    if (!kShare && in_block) {
+                };
+                // the update of dim p (flux, sources, gravity) in the reference order: dims in turn
+                auto accumulate = [&](int p, const State<ND, T>& FL, const State<ND, T>& FR, const X& XL, const X& XR) {
+                    T const dS = T(face_area<ND, GEOM>(G, p, ci, cj, ck));
+                    T const dSp = T(face_area<ND, GEOM>(G, p, ci + (p == 0), cj + (p == 1), ck));
 #pragma unroll
-                        for (int v = 0; v < NV; ++v) s_fl[p][v][hy][hx] = F[v];
+                    for (int v = 0; v < NV; ++v) part[v] += dtodv * ((FL[v] * dS) - (FR[v] * dSp));
+                    godunov_sources<Eos, ND, GEOM>(G, eos, p, ci, cj, dtodv, dS, dSp, fs[0][p], fs[1][p], part);
+                    T const g = T(gravity(G, ci, p));
+                    part[1 + p] += dt * g * c0[0];
+                    part[ND + 1] += dt * g * c0[1 + p];
 #pragma unroll
-                        for (int f = 0; f < NFX; ++f) s_fl[p][NV + f][hy][hx] = XF.f[f];
-                    }
-                }
-                if constexpr (kShare) {  // every left state is read before the buffer takes the fluxes
-                    __syncthreads();
+                    for (int f = 0; f < NFX; ++f) xpart.f[f] += dtodv * ((XL.f[f] * dS) - (XR.f[f] * dSp));
+                };
+                // y (3D): right face states through LDS; s_st was last read before the flux
+                // barrier of the previous plane
+                if constexpr (ND == 3) {
                     if (in_block) {
 #pragma unroll
-                        for (int p = 0; p < NP; ++p) {
+                        for (int v = 0; v < NV; ++v) s_st[0][v][hy][hx] = fs[1][1][v];
 #pragma unroll
-                            for (int v = 0; v < NV; ++v) s_fl[p][v][hy][hx] = FLp[p][v];
-#pragma unroll
-                            for (int f = 0; f < NFX; ++f) s_fl[p][NV + f][hy][hx] = XLp[p].f[f];
-                        }
+                        for (int f = 0; f < NFX; ++f) s_st[0][NV + f][hy][hx] = xf[1][1].f[f];
                     }
                 }
-                __syncthreads();
-                if (out_col) {
+                {  // x: the column at x - 1 is the lane beside (every lane executes the shifts)
+                    State<ND, T> left;
+                    X xleft;
 #pragma unroll
-                    for (int p = 0; p < NP; ++p) {
+                    for (int v = 0; v < NV; ++v) left[v] = lane_shift<ND, true>(fs[1][0][v]);
 #pragma unroll
-                        for (int v = 0; v < NV; ++v) FRp[p][v] = (p == 0) ? s_fl[p][v][hy][hx + 1] : s_fl[p][v][hy + 1][hx];
+                    for (int f = 0; f < NFX; ++f) xleft.f[f] = lane_shift<ND, true>(xf[1][0].f[f]);
+                    bool const need = hancock_ok && hx >= 1 && (ND == 2 || (hy >= 1 && hy <= TY));
+                    State<ND, T> FL, FR;
+                    X XL, XR;
+                    solve(0, need, left, xleft, FL, XL);
+                    // the right face's flux is the left flux of the column at x + 1
 #pragma unroll
-                        for (int f = 0; f < NFX; ++f) XRp[p].f[f] = (p == 0) ? s_fl[p][NV + f][hy][hx + 1] : s_fl[p][NV + f][hy + 1][hx];
+                    for (int v = 0; v < NV; ++v) FR[v] = lane_shift<ND, false>(FL[v]);
+#pragma unroll
+                    for (int f = 0; f < NFX; ++f) XR.f[f] = lane_shift<ND, false>(XL.f[f]);
+                    if (out_col) accumulate(0, FL, FR, XL, XR);
+                }
+                if constexpr (ND == 3) {  // y: left states and right fluxes through LDS
+                    __syncthreads();
+                    bool const need = hancock_ok && hy >= 1 && hx >= 1 && hx <= TX;
+                    State<ND, T> left, FL, FR;
+                    X xleft = zero_frac<NFX, T>(), XL, XR;
+                    if (need) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) left[v] = s_st[0][v][hy - 1][hx];
+#pragma unroll
+                        for (int f = 0; f < NFX; ++f) xleft.f[f] = s_st[0][NV + f][hy - 1][hx];
                     }
+                    solve(1, need, left, xleft, FL, XL);
+                    // s_fl was last read before the primitive barrier of this plane
+                    if (in_block) {
 #pragma unroll
-                    for (int p = 0; p < NP; ++p) {
-                        T const dS = T(face_area<ND, GEOM>(G, p, ci, cj, ck));
-                        T const dSp = T(face_area<ND, GEOM>(G, p, ci + (p == 0), cj + (p == 1), ck));
+                        for (int v = 0; v < NV; ++v) s_fl[0][v][hy][hx] = FL[v];
 #pragma unroll
-                        for (int v = 0; v < NV; ++v) part[v] += dtodv * ((FLp[p][v] * dS) - (FRp[p][v] * dSp));
-                        godunov_sources<Eos, ND, GEOM>(G, eos, p, ci, cj, dtodv, dS, dSp, fs[0][p], fs[1][p], part);
-                        T const g = T(gravity(G, ci, p));
-                        part[1 + p] += dt * g * c0[0];
-                        part[ND + 1] += dt * g * c0[1 + p];
+                        for (int f = 0; f < NFX; ++f) s_fl[0][NV + f][hy][hx] = XL.f[f];
+                    }
+                    __syncthreads();
+                    if (out_col) {
 #pragma unroll
-                        for (int f = 0; f < NFX; ++f) xpart.f[f] += dtodv * ((XLp[p].f[f] * dS) - (XRp[p].f[f] * dSp));
+                        for (int v = 0; v < NV; ++v) FR[v] = s_fl[0][v][hy + 1][hx];
+#pragma unroll
+                        for (int f = 0; f < NFX; ++f) XR.f[f] = s_fl[0][NV + f][hy + 1][hx];
+                        accumulate(1, FL, FR, XL, XR);
                     }
                 }
             }
