@@ -30,6 +30,12 @@ one JSON line:
              ratio to FP64, and the relative L1 distance of its density to the FP64
              density after the configured step count (500 for Rayleigh-Taylor;
              both runs continue untimed to it)
+  fast_fp64  the same workload with the opt-in fast FP64 arithmetic (the reference's
+             formulas without its rounding; hclpp_set_fast_arithmetic): Gcell/s, kernel
+             time, ratio to the reference arithmetic, density L1 to the reference run
+             after the configured step count, and relative-L1 divergence curves against
+             the reference arithmetic (Rayleigh-Taylor 128^3 to 500 steps, Kelvin-Helmholtz
+             2048^2 to 2000 steps)
   kelvin_helmholtz_2048  the 2D Kelvin-Helmholtz 2048^2 run (VanLeer + HLLC) in
              FP64 and with FP32 storage, timed the same way (default workload only)
   adjoint    differentiable-Euler 1024^2 (VanAlbada + HLLC, 400 steps,
@@ -219,7 +225,8 @@ def calibrate_copy(nbytes: int):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
 
 
-def _timed_run(workload: str, precision: str, steps: int, warmup: int, warmup_seconds: float, total: int):
+def _timed_run(workload: str, precision: str, steps: int, warmup: int, warmup_seconds: float, total: int,
+               arithmetic: str = "reference"):
     """Fused steps of a workload with the state resident in HBM: `warmup` untimed steps, more until
     `warmup_seconds` have passed (at most `total` - `steps` in all), then exactly `steps` timed steps
     between synchronisations, then untimed steps up to `total` (if more).  Returns a record: the setup
@@ -238,7 +245,7 @@ def _timed_run(workload: str, precision: str, steps: int, warmup: int, warmup_se
     grid = Grid(s.ndim, s.n, s.ng, s.geom, s.xmin, s.xmax)
     solver = HydroSolver(grid, PerfectGas(s.gamma, 1.0), reconstruction=s.reconstruction,
                          riemann_solver=s.riemann_solver, gravity=UniformGravity(s.gravity), bc=s.bc, cfl=s.cfl,
-                         precision=precision)
+                         precision=precision, arithmetic=arithmetic)
     solver.init_from_prim(s.rho, s.u, s.P)
     del s.rho, s.u, s.P
     cells = 1
@@ -306,6 +313,72 @@ def mixed_block(workload: str, a, value64: float, rho64, total: int):
             "warmup_steps_run": r.warmup_steps}
 
 
+def divergence_curve(workload: str, kw: dict, total: int, every: int):
+    """The fast FP64 arithmetic against the reference arithmetic (bit-identical to the oracle) on the
+    same initial state: relative L1 distance of rho, rhou and E every `every` steps up to `total`, and
+    the first checkpoint at which each exceeds 1e-12."""
+    import torch
+
+    from heraclespp_amd.hydro import Grid, PerfectGas, UniformGravity
+    from heraclespp_amd.solver import HydroSolver
+
+    s = _setup(workload, kw)
+    grid = Grid(s.ndim, s.n, s.ng, s.geom, s.xmin, s.xmax)
+    sols = []
+    for arith in ("reference", "fast"):
+        sol = HydroSolver(grid, PerfectGas(s.gamma, 1.0), reconstruction=s.reconstruction, riemann_solver=s.riemann_solver,
+                          gravity=UniformGravity(s.gravity), bc=s.bc, cfl=s.cfl, arithmetic=arith)
+        sol.init_from_prim(s.rho, s.u, s.P)
+        sol.start_clock(1e300, total)
+        sols.append(sol)
+    I = (Ellipsis,) + s.interior()
+    curve, first = [], {}
+    for n in range(every, total + 1, every):
+        for sol in sols:
+            for _ in range(every):
+                sol.step_async()
+        torch.cuda.synchronize()
+        # the state after `launched` steps: buffer (cur0 + launched) % 2 (no early stop before `total`)
+        cur = [sol._bufs()[(sol._cur0 + sol._launched) % 2] for sol in sols]
+        row = {"n": n}
+        for k, nm in enumerate(("rho", "rhou", "E")):
+            ref, fast = cur[0][k][I], cur[1][k][I]
+            d = float((fast - ref).abs().sum() / ref.abs().sum())
+            row[nm] = float(f"{d:.3e}")
+            if d > 1e-12 and nm not in first:
+                first[nm] = n
+        curve.append(row)
+    for sol in sols:
+        clk = sol.finish()
+        if clk.iter != total:
+            raise RuntimeError(f"expected {total} steps, clock counted {clk.iter}")
+    del sols
+    torch.cuda.empty_cache()
+    return {"workload": f"{workload} {'x'.join(str(v) for v in s.n[:s.ndim])}", "rel_l1": curve,
+            "first_n_above_1e-12": {nm: first.get(nm) for nm in ("rho", "rhou", "E")}}
+
+
+def fast_block(workload: str, a, value_ref: float, rho_ref, total: int):
+    """The workload with the opt-in fast FP64 arithmetic (hclpp_set_fast_arithmetic): throughput timed
+    as the reference line, its density's relative L1 distance to the reference run after the same
+    `total` steps, and divergence curves against the reference arithmetic on Rayleigh-Taylor 128^3
+    (500 steps) and Kelvin-Helmholtz 2048^2 (2000 steps)."""
+    r = _timed_run(workload, "fp64", a.steps, a.warmup, a.warmup_seconds, total, arithmetic="fast")
+    gbs, frac = _hbm(r, "fp64")
+    out = {"value": round(r.value, 4), "unit": "Gcell-updates/s", "kernel_ms": round(r.kernel_ms, 5),
+           "hbm_achieved_gbs": gbs, "hbm_frac": frac, "ratio_to_reference": round(r.value / value_ref, 3),
+           "density_rel_l1_vs_reference": float(f"{_rel_l1(r.rho, rho_ref):.3e}"), "steps_compared": r.total,
+           "launch": r.plan, "warmup_steps_run": r.warmup_steps}
+    units = _valu_units(workload, "fp64_fast")
+    if units is not None:
+        out["valu"] = {"issue_units_per_cell": units, "roof_gcells": round(39.3e12 / units / 1e9, 3),
+                       "source": f"profiles/k_step_march_{workload}_fp64_fast_valu.txt"}
+    del r
+    out["divergence"] = [divergence_curve("rayleigh_taylor", {"n": 128}, 500, 50),
+                         divergence_curve("kelvin_helmholtz", {"n": 2048}, 2000, 200)]
+    return out
+
+
 def kh_block(a):
     """The 2D Kelvin-Helmholtz 2048^2 run (VanLeer + HLLC) in FP64 and with FP32 storage: throughput,
     kernel time and HBM fraction at the algorithmic bytes, and the FP32/FP64 density L1 after its
@@ -338,6 +411,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-Euler adjoint measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-storage run of the same workload")
     ap.add_argument("--no-kh", action="store_true", help="skip the 2D Kelvin-Helmholtz 2048^2 FP64 / FP32 block")
+    ap.add_argument("--no-fast", action="store_true", help="skip the fast-FP64-arithmetic block")
+    ap.add_argument("--arithmetic", default="reference", choices=["reference", "fast"],
+                    help="fast: the headline run with the opt-in fast FP64 arithmetic (not bit-identical to the reference)")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="stop after the timed steps instead of continuing to the configured step count")
     ap.add_argument("--adjoint-n", type=int, default=1024)
@@ -356,8 +432,10 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
-    total = 0 if (a.no_accuracy or a.no_mixed or a.precision != "fp64") else WORKLOADS[a.workload][3]
-    r = _timed_run(a.workload, a.precision, a.steps, a.warmup, a.warmup_seconds, total)
+    if a.arithmetic == "fast" and a.precision != "fp64":
+        ap.error("--arithmetic fast is an FP64 mode")
+    total = 0 if (a.no_accuracy or (a.no_mixed and a.no_fast) or a.precision != "fp64") else WORKLOADS[a.workload][3]
+    r = _timed_run(a.workload, a.precision, a.steps, a.warmup, a.warmup_seconds, total, arithmetic=a.arithmetic)
     s, cells = r.setup, r.cells
     wall = r.wall
     value = r.value
@@ -366,8 +444,9 @@ def main():
     bytes_alg = 2 * (s.ndim + 2) * word * cells
     achieved = bytes_alg / kernel_s / 1e9
     hbm_frac = achieved / HBM_PEAK_GBS
-    traffic = _traffic(a.workload, a.precision)
-    units = _valu_units(a.workload, a.precision)
+    tag = a.precision + ("_fast" if a.arithmetic == "fast" else "")
+    traffic = _traffic(a.workload, tag)
+    units = _valu_units(a.workload, tag)
     # FP64 VALU issue capacity: 1024 SIMDs, one wave64 FP64 instruction per 4 cycles each, at 2.4 GHz
     valu = None
     bound = "hbm"
@@ -378,7 +457,7 @@ def main():
         valu = {"issue_units_per_cell": units, "busy_frac": round(issued / capacity, 4),
                 "roof_gcells": round(roof, 3), "kernel_gcells": round(cells / kernel_s / 1e9, 4),
                 "frac_of_roof": round(cells / kernel_s / 1e9 / roof, 4),
-                "source": f"profiles/k_step_march_{a.workload}_{a.precision}_valu.txt"}
+                "source": f"profiles/k_step_march_{a.workload}_{tag}_valu.txt"}
         if issued / capacity > hbm_frac:
             bound = "valu"
     line = {
@@ -401,6 +480,7 @@ def main():
             "reconstruction": s.reconstruction,
             "riemann_solver": s.riemann_solver,
             "storage": a.precision,
+            "arithmetic": a.arithmetic,
             "path": "fused (hclpp_step: march kernel + ghost fill + device dt)",
             "launch": r.plan,
             "warmup_steps_run": r.warmup_steps,
@@ -421,15 +501,20 @@ def main():
         },
         "cpu_baseline": None,
         "mixed_fp32": None,
+        "fast_fp64": None,
         "kelvin_helmholtz_2048": None,
         "adjoint": None,
     }
     rho64, total = r.rho, r.total
     del r
-    if a.precision == "fp64" and not a.no_mixed:
+    reference_fp64 = a.precision == "fp64" and a.arithmetic == "reference"
+    if reference_fp64 and not a.no_mixed:
         line["mixed_fp32"] = mixed_block(a.workload, a, value, rho64, total)
         torch.cuda.empty_cache()
-    if a.precision == "fp64" and a.workload == "rayleigh_taylor" and not a.no_kh:
+    if reference_fp64 and not a.no_fast:
+        line["fast_fp64"] = fast_block(a.workload, a, value, rho64, total)
+        torch.cuda.empty_cache()
+    if reference_fp64 and a.workload == "rayleigh_taylor" and not a.no_kh:
         line["kelvin_helmholtz_2048"] = kh_block(a)
     if not a.no_adjoint:
         line["adjoint"] = adjoint_block(a.adjoint_n, a.adjoint_steps, 20, a.adjoint_fd_dirs)

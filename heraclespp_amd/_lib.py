@@ -42,6 +42,7 @@ EXPORTS = [
     "hclpp_profile_begin", "hclpp_profile_end", "hclpp_fused_plan", "hclpp_time_step_noalloc",
     "hclpp_clock_set_outputs", "hclpp_clock_resume", "hclpp_clock_resume_f32", "hclpp_temperature",
     "hclpp_set_pressure_fix",
+    "hclpp_set_fast_arithmetic",
 ]
 
 
@@ -133,6 +134,7 @@ def lib() -> ctypes.CDLL:
         "hclpp_clock_read": [vp, ctypes.POINTER(Clock)],
         "hclpp_clock_set_outputs": [vp, ip, d, d, ip, ip],
         "hclpp_set_pressure_fix": [vp, ip, d],
+        "hclpp_set_fast_arithmetic": [vp, ip],
         "hclpp_clock_resume": [vp, dp, dp, dp],
         "hclpp_clock_resume_f32": [vp, dp, dp, dp],
         "hclpp_temperature": [vp, rp, dp, dp, dp],

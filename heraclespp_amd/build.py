@@ -42,8 +42,12 @@ LIB_IEEE = os.path.join(HERE, "libhclpp_hip_ieee.so")
 LIB_IO = os.path.join(HERE, "libhclpp_io.so")
 HDF5_ROOT = os.environ.get("HCLPP_HDF5_ROOT", "/opt/conda")
 
-SOURCES = ["ref_kernels.hip", "step_fused.hip", "step_fused_fx.hip", "pressure_fix_fused.hip", "adjoint.hip", "abi.cpp"]
-EXTRA = {src: ["-mllvm", "-amdgpu-sched-strategy=max-ilp"] for src in ("step_fused.hip", "step_fused_fx.hip")}
+SOURCES = ["ref_kernels.hip", "step_fused.hip", "step_fused_fx.hip", "step_fused_fast.hip", "pressure_fix_fused.hip", "adjoint.hip",
+           "abi.cpp"]
+EXTRA = {src: ["-mllvm", "-amdgpu-sched-strategy=max-ilp"] for src in ("step_fused.hip", "step_fused_fx.hip", "step_fused_fast.hip")}
+# the fast FP64 arithmetic is not bound to the reference's rounding: let the compiler fuse
+# multiplies and adds there
+EXTRA["step_fused_fast.hip"] += ["-ffp-contract=fast"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

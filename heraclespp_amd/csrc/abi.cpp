@@ -35,6 +35,8 @@ struct hclpp_ctx {
     GridDev G;
     PerfectGas eos;
     PerfectGasMixed eos32;
+    PerfectGasFast eos_fast;
+    int fast_on = 0;  // hclpp_set_fast_arithmetic: the FP64 fused step uses eos_fast
     BcSpec bc;
     hipStream_t stream = nullptr;
     int device = 0;
@@ -272,6 +274,9 @@ int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out)
     c->eos32.inv_gamma_m1 = (float)(1.0 / (cfg->gamma - 1));
     c->eos32.gamma_m1 = cfg->gamma - 1;
     c->eos32.gamma_d = cfg->gamma;
+    c->eos_fast.gamma = cfg->gamma;
+    c->eos_fast.gamma_m1 = cfg->gamma - 1;
+    c->eos_fast.inv_gamma_m1 = 1.0 / (cfg->gamma - 1);
     for (int f = 0; f < 6; ++f) c->bc.kind[f] = f < 2 * nd ? cfg->bc[f] : 0;
 
     // host tables (mesh/grid.cpp:31-44, :134-155; geometry.cpp trigonometry; gravity.cpp:44-55)
@@ -360,7 +365,7 @@ int hclpp_ctx_create(const hclpp_config* cfg, hclpp_ctx** out)
         e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return cleanup(hip_fail(e, "hipDeviceGetAttribute(multiProcessorCount)"));
-        int const occ = march_occupancy(nd, cfg->geom, cfg->riemann, cfg->limiter, cfg->precision == HCLPP_FP32_STORAGE, cfg->nfx == 1);
+        int const occ = march_occupancy(nd, cfg->geom, cfg->riemann, cfg->limiter, cfg->precision == HCLPP_FP32_STORAGE, cfg->nfx >= 1);
         c->march = plan_march(G, nd, occ, ncu);
         c->march_occ = occ;
     }
@@ -651,10 +656,9 @@ static int fused_step(hclpp_ctx* c, const S* rho, const S* rhou, const S* E, con
     bool const with_fx = fx || fx_new;
     if (!with_fx && nfx > 0)
         return fail(HCLPP_ERR_UNSUPPORTED, f32 ? "nfx > 0: use hclpp_step_fx_f32" : "nfx > 0: use hclpp_step_fx (or the operator entry points)");
-    if (with_fx && nfx > 1)
-        return fail(HCLPP_ERR_UNSUPPORTED, "the fused step carries at most one passive scalar; use the operator entry points");
     if (f32 && c->cfg.precision != HCLPP_FP32_STORAGE) return fail(HCLPP_ERR_ARG, "the context was not created for FP32 storage");
     if (!f32 && c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "hclpp_step takes FP64 storage");
+    if (with_fx && c->fast_on) return fail(HCLPP_ERR_UNSUPPORTED, "the fast FP64 arithmetic carries no passive scalars");
     if (c->cfg.ng < 2) return fail(HCLPP_ERR_ARG, "the fused step needs ng >= 2 (two-cell reconstruction stencil)");
     if (!rho || !rhou || !E || !rho_new || !rhou_new || !E_new) return fail(HCLPP_ERR_ARG, "null array");
     if (nfx > 0 && (!fx || !fx_new)) return fail(HCLPP_ERR_ARG, "null passive scalar array");
@@ -670,6 +674,7 @@ static int fused_step(hclpp_ctx* c, const S* rho, const S* rhou, const S* E, con
     A.fx = nfx > 0 ? fx : nullptr;
     A.fx_new = nfx > 0 ? fx_new : nullptr;
     A.nfx = nfx;
+    A.scalars_only = 0;
     A.pf = f32 ? 0 : c->pf_on;
     A.eps_pf = c->pf_eps;
     A.partials = c->partials;
@@ -686,8 +691,26 @@ static int fused_step(hclpp_ctx* c, const S* rho, const S* rhou, const S* E, con
     }
     bool const timed = c->prof_count < c->prof_max;
     if (timed) HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count], c->stream));
-    if (nfx > 0) launch_fused_step_fx(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->march, eos, A, c->stream);
-    else launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->march, eos, A, c->stream);
+    if (nfx > 0) {
+        // one launch of the one-scalar kernel per passive scalar (the scalars do not couple: each
+        // is upwinded with the same mass flux, godunov_scheme.hpp:263-301); the first also writes
+        // the hydro update and the partials
+        for (int f = 0; f < nfx; ++f) {
+            StepArraysT<S> Af = A;
+            Af.fx = fx + c->G.plane * f;
+            Af.fx_new = fx_new + c->G.plane * f;
+            Af.nfx = 1;
+            Af.scalars_only = f > 0;
+            launch_fused_step_fx(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->march, eos, Af, c->stream);
+        }
+    } else {
+        bool fast = false;
+        if constexpr (!f32) fast = c->fast_on;
+        if constexpr (!f32) {
+            if (fast) launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->march, c->eos_fast, A, c->stream);
+        }
+        if (!fast) launch_fused_step(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->march, eos, A, c->stream);
+    }
     HCLPP_LAUNCHED("fused step");
     if (timed) {
         HCLPP_HIP(hipEventRecord(c->prof_events[2 * c->prof_count + 1], c->stream));
@@ -740,6 +763,7 @@ int hclpp_set_pressure_fix(hclpp_ctx* c, int on, double eps)
 {
     if (!c) return fail(HCLPP_ERR_ARG, "null context");
     if (on && c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "the pressure fix runs with FP64 storage only");
+    if (on && c->fast_on) return fail(HCLPP_ERR_UNSUPPORTED, "the pressure fix runs with the reference arithmetic only");
     if (on && !std::isfinite(eps)) return fail(HCLPP_ERR_ARG, "eps must be finite");
     if (on && !c->fix_partials) {
         int64_t const nb = reduce_blocks(interior_range(c->G));
@@ -748,6 +772,36 @@ int hclpp_set_pressure_fix(hclpp_ctx* c, int on, double eps)
     }
     c->pf_on = on ? 1 : 0;
     c->pf_eps = eps;
+    return HCLPP_OK;
+}
+
+// Launch plan of the fused kernel the context runs (the plan depends on its occupancy) and the
+// partials buffer it fills.  Waits for the stream: the buffer may be reallocated.
+static int replan_march(hclpp_ctx* c, int occ)
+{
+    int dev = 0, ncu = 0;
+    HCLPP_HIP(hipGetDevice(&dev));
+    HCLPP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HCLPP_HIP(hipStreamSynchronize(c->stream));
+    c->march = plan_march(c->G, c->cfg.ndim, occ, ncu);
+    c->march_occ = occ;
+    c->fused_nb = c->march.nb;
+    return ensure_partials(c, c->fused_nb * kPartialStride);
+}
+
+int hclpp_set_fast_arithmetic(hclpp_ctx* c, int on)
+{
+    if (!c) return fail(HCLPP_ERR_ARG, "null context");
+    if (on && c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "the fast arithmetic is an FP64-storage mode");
+    if (on && c->cfg.nfx > 0) return fail(HCLPP_ERR_UNSUPPORTED, "the fast FP64 arithmetic carries no passive scalars");
+    if (on && c->pf_on) return fail(HCLPP_ERR_UNSUPPORTED, "the pressure fix runs with the reference arithmetic only");
+    if ((on != 0) == (c->fast_on != 0)) return HCLPP_OK;
+    int const nd = c->cfg.ndim;
+    int const occ = on ? march_occupancy_fast(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter)
+                       : march_occupancy(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, false, false);
+    int const rc = replan_march(c, occ);
+    if (rc != HCLPP_OK) return rc;
+    c->fast_on = on ? 1 : 0;
     return HCLPP_OK;
 }
 

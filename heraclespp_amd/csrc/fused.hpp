@@ -40,6 +40,10 @@ struct StepArraysT {
     const S* __restrict__ fx;  // passive scalars (mass fractions, [f][cell]); unused when nfx = 0
     S* __restrict__ fx_new;
     int nfx;
+    // launches after the first of a step with several passive scalars (one scalar per launch, fx
+    // and fx_new pointing at it): the kernel writes that scalar's update only, not the hydro
+    // update or the workgroup partials the first launch wrote
+    int scalars_only;
     // pressure fix (FP64 only): cells whose state at time n has E - e_k < eps_pf e_k are left out
     // of the march kernel's CFL maximum; k_pressure_fix_fused rewrites their energy and rate
     int pf;
@@ -92,13 +96,17 @@ struct MarchPlan {
 // per CU, on ncu CUs (see step_fused.hip for the cost model).
 MarchPlan plan_march(const GridDev& G, int nd, int blocks_per_cu, int ncu);
 // Resident workgroups per CU of the k_step_march instance a configuration selects
-// (FP32 storage or not, one passive scalar or none).
+// (FP32 storage or not, with the one-scalar kernel or without scalars).
 int march_occupancy(int nd, int geom, int rs, int lim, bool fp32, bool fx);
 void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGas& eos, const StepArrays& A,
                        hipStream_t s);
 void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGasMixed& eos,
                        const StepArraysF& A, hipStream_t s);
-// the fused step with one passive scalar (step_fused_fx.hip); A.fx and A.fx_new are set
+// the fused step with the opt-in fast FP64 arithmetic (step_fused_fast.hip; no passive scalars)
+void launch_fused_step(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGasFast& eos, const StepArrays& A,
+                       hipStream_t s);
+int march_occupancy_fast(int nd, int geom, int rs, int lim);
+// the fused step carrying one passive scalar (step_fused_fx.hip); A.fx and A.fx_new point at it
 void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGas& eos, const StepArrays& A,
                           hipStream_t s);
 void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGasMixed& eos,

@@ -29,7 +29,8 @@
 //    conserved totals; per-workgroup partials are folded in a fixed order.
 //  * The kernels are generic over the precision policy (physics.hpp):
 //    PerfectGas streams FP64; PerfectGasMixed streams FP32 (half the bytes)
-//    with FP64 pressure recovery and FP64 reductions.
+//    with FP64 pressure recovery and FP64 reductions; PerfectGasFast streams
+//    FP64 with the reference's formulas evaluated without its rounding.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -60,30 +61,31 @@ __device__ __forceinline__ void ijk_of(int a, int b, int m, int& i, int& j, int&
     else { i = a; j = b; k = m; }
 }
 
-// Register exchange with the neighbouring column along x: a tile row is a DPP row (3D: rows of
-// 16 lanes, row_shr / row_shl) or the whole wave (2D: 64 columns, wave_shr / wave_shl), so the
-// column at x - 1 (FROM_LEFT) or x + 1 is the lane beside this one.  Every lane of the wave must
-// execute the shift; the first (FROM_LEFT) or last lane of a row receives 0.
-template <int ND, bool FROM_LEFT>
+// Register exchange with the neighbouring column along x.  A tile row of ROW columns occupies
+// ROW consecutive lanes (3D: 16, a DPP row, row_shr / row_shl; 2D: 64, the whole wave, wave_shr /
+// wave_shl), so the column at x - 1 (FROM_LEFT) or x + 1 is the lane beside this one.  Every lane
+// of the wave must execute the shift; the first (FROM_LEFT) or last lane of a row receives 0.
+template <int ROW, bool FROM_LEFT>
 __device__ __forceinline__ int lane_shift_u32(int v)
 {
-    constexpr int kCtrl = ND == 2 ? (FROM_LEFT ? 0x138 : 0x130) : (FROM_LEFT ? 0x111 : 0x101);
+    static_assert(ROW == 16 || ROW == 64, "tile rows of 16 or 64 columns");
+    constexpr int kCtrl = ROW == 16 ? (FROM_LEFT ? 0x111 : 0x101) : (FROM_LEFT ? 0x138 : 0x130);
     return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, true);
 }
 
-template <int ND, bool FROM_LEFT>
+template <int ROW, bool FROM_LEFT>
 __device__ __forceinline__ double lane_shift(double v)
 {
     int2 p = __builtin_bit_cast(int2, v);
-    p.x = lane_shift_u32<ND, FROM_LEFT>(p.x);
-    p.y = lane_shift_u32<ND, FROM_LEFT>(p.y);
+    p.x = lane_shift_u32<ROW, FROM_LEFT>(p.x);
+    p.y = lane_shift_u32<ROW, FROM_LEFT>(p.y);
     return __builtin_bit_cast(double, p);
 }
 
-template <int ND, bool FROM_LEFT>
+template <int ROW, bool FROM_LEFT>
 __device__ __forceinline__ float lane_shift(float v)
 {
-    return __builtin_bit_cast(float, lane_shift_u32<ND, FROM_LEFT>(__builtin_bit_cast(int, v)));
+    return __builtin_bit_cast(float, lane_shift_u32<ROW, FROM_LEFT>(__builtin_bit_cast(int, v)));
 }
 
 template <int ND, class S>
@@ -107,11 +109,12 @@ __device__ __forceinline__ void store_cons(const StepArraysT<S>& A, const GridDe
 }
 
 // CFL rate of a conservative state at (i,j,k) (time_step.hpp:61-76), in FP64.  With the
-// FP64 policy this is the reference's arithmetic (conv_cons_to_prim, then the rate).
+// FP64 policies this is the policy's conv_cons_to_prim, then the rate (the reference's
+// arithmetic for PerfectGas).
 template <int ND, int GEOM, class Eos>
 __device__ __forceinline__ double state_rate(const GridDev& G, const Eos& eos, int i, int j, int k, const State<ND, typename Eos::real>& c)
 {
-    if constexpr (std::is_same_v<Eos, PerfectGas>) {
+    if constexpr (kFp64State<Eos>) {
         State<ND> const p = cons_to_prim(c, eos);
         return cfl_rate<ND, GEOM>(G, i, j, k, &p.q[1], eos.sound(p[0], p[ND + 1]));
     } else {
@@ -136,7 +139,7 @@ __device__ __forceinline__ double state_eint(const Eos& eos, const State<ND, typ
 template <int LIM, class Eos, class T>
 __device__ __forceinline__ void reconstruct(T vm, T v, T vp, T dlm, T dl, T dlp, T& lo, T& hi)
 {
-    T const slope = limited_slope_t<LIM, Eos, T>(Eos::div(vp - v, (dl + dlp) * T(0.5)), Eos::div(v - vm, (dlm + dl) * T(0.5)));
+    T const slope = slope_limiter<LIM, Eos, T>(Eos::div(vp - v, (dl + dlp) * T(0.5)), Eos::div(v - vm, (dlm + dl) * T(0.5)));
     lo = v - ((dl * T(0.5)) * slope);
     hi = v + ((dl * T(0.5)) * slope);
 }
@@ -335,7 +338,7 @@ template <class Eos, int ND, int GEOM>
 constexpr int march_min_waves()
 {
     if constexpr (ND != 3) return 1;
-    else if constexpr (!std::is_same_v<Eos, PerfectGas>) return 3;
+    else if constexpr (!kFp64State<Eos>) return 3;
     else return GEOM == kCartesian ? 2 : 1;
 }
 
@@ -381,7 +384,7 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
     // budget is set for 3 waves per SIMD (spills go from 40-152 to 0-92 B per lane; RT 512^3
     // +6 %, spherical Sedov +26 %), and the 3D Cartesian FP64 kernel run at 2 waves per SIMD.
     // In spherical FP64 and in 2D it gained nothing or lost.
-    constexpr bool kPark = ND == 3 && (!std::is_same_v<Eos, PerfectGas> || GEOM == kCartesian);
+    constexpr bool kPark = ND == 3 && (!kFp64State<Eos> || GEOM == kCartesian);
     constexpr int NPARK = kPark ? 3 * NV + 3 * NFX : 1;
     __shared__ T s_park[NPARK][kPark ? NT : 1];
 
@@ -490,7 +493,7 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
     // and cp that their pressure recovery forms, and dt / dV of plane h-1 for its completion.
     // The 3D Cartesian FP64 kernel (2 waves per SIMD) and the 3D spherical FP32 one recompute
     // them instead: the extra registers spill there.
-    constexpr bool kCarry = !(ND == 3 && (std::is_same_v<Eos, PerfectGas> == (GEOM == kCartesian)));
+    constexpr bool kCarry = !(ND == 3 && (kFp64State<Eos> == (GEOM == kCartesian)));
     double e0 = 1, ep = 1;
     T dtodv_prev = 0;
     State<ND, T> fsR_prev, Fm_prev, part_prev;  // in registers, or in s_park when kPark
@@ -711,18 +714,18 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
                     State<ND, T> left;
                     X xleft;
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) left[v] = lane_shift<ND, true>(fs[1][0][v]);
+                    for (int v = 0; v < NV; ++v) left[v] = lane_shift<HX, true>(fs[1][0][v]);
 #pragma unroll
-                    for (int f = 0; f < NFX; ++f) xleft.f[f] = lane_shift<ND, true>(xf[1][0].f[f]);
+                    for (int f = 0; f < NFX; ++f) xleft.f[f] = lane_shift<HX, true>(xf[1][0].f[f]);
                     bool const need = hancock_ok && hx >= 1 && (ND == 2 || (hy >= 1 && hy <= TY));
                     State<ND, T> FL, FR;
                     X XL, XR;
                     solve(0, need, left, xleft, FL, XL);
                     // the right face's flux is the left flux of the column at x + 1
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) FR[v] = lane_shift<ND, false>(FL[v]);
+                    for (int v = 0; v < NV; ++v) FR[v] = lane_shift<HX, false>(FL[v]);
 #pragma unroll
-                    for (int f = 0; f < NFX; ++f) XR.f[f] = lane_shift<ND, false>(XL.f[f]);
+                    for (int f = 0; f < NFX; ++f) XR.f[f] = lane_shift<HX, false>(XL.f[f]);
                     if (out_col) accumulate(0, FL, FR, XL, XR);
                 }
                 if constexpr (ND == 3) {  // y: left states and right fluxes through LDS
@@ -806,7 +809,7 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
                 T const g = T(gravity(G, pi, MD));
                 un[1 + MD] += dt * g * old_rho_prev;
                 un[ND + 1] += dt * g * old_mom_prev;
-                store_cons<ND>(A, G, cell_at<ND>(G, a, bb, hm), un);
+                if (!A.scalars_only) store_cons<ND>(A, G, cell_at<ND>(G, a, bb, hm), un);
                 if constexpr (NFX > 0) {  // godunov_scheme.hpp:288, then passive_scalar_Godunov (:294-301)
                     int64_t const id = cell_at<ND>(G, a, bb, hm);
 #pragma unroll
@@ -872,7 +875,7 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
     r_emin = block_reduce<kMin>(r_emin, s_red, tid, NT);
 #pragma unroll
     for (int v = 0; v < NV; ++v) r_sum[v] = block_reduce<kSum>(r_sum[v], s_red, tid, NT);
-    if (tid == 0) {
+    if (tid == 0 && !A.scalars_only) {
         double* out = A.partials + (int64_t)blockIdx.x * kPartialStride;
         out[0] = r_inv;
         out[1] = r_emin;

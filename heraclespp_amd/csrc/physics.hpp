@@ -10,7 +10,12 @@
 //    -ffp-contract=off), so results are bit-identical to the reference;
 //  * PerfectGasMixed (FP32): state and fluxes in FP32 with hardware reciprocal
 //    divisions, while the pressure recovery P = (gamma-1)(E - |m|^2 / 2 rho),
-//    which cancels at high Mach number, is evaluated in FP64.
+//    which cancels at high Mach number, is evaluated in FP64;
+//  * PerfectGasFast (FP64, opt-in): the same FP64 state and algorithm without
+//    the reference's rounding -- P = (gamma-1) e directly instead of the
+//    temperature round trip through kb and mp, divisions as a product with a
+//    shared reciprocal, and the slope limiters in closed form.  Not a parity
+//    path: its distance to the reference grows with the step count.
 // Division is always written through Eos::div (or fdiv below) so the FP64
 // policy keeps the correctly-rounded quotient (on normal-range operands, see
 // fdiv) and the FP32 policy uses v_rcp_f32.  Halving is a multiplication by
@@ -21,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 namespace hclpp {
 
@@ -188,6 +194,59 @@ struct PerfectGasMixed {
     }
 };
 
+// FP64 state and arithmetic, without the reference's rounding (opt-in).  Each formula is the
+// reference's up to rounding:
+//   * pressure and internal energy: P = (gamma-1) e and e = P / (gamma-1) (perfect_gas.hpp:46-78
+//     go through the temperature, T = e mmw mp (gamma-1) / (rho kb) and P = rho kb T / (mmw mp),
+//     whose constants cancel);
+//   * a / b = a * (1 / b), the reciprocal from v_rcp_f64 and two Newton steps (within about two
+//     ulp of the quotient for normal operands); every division by one divisor (1 / rho in the
+//     velocities and the kinetic energy, 1 / (wsR - wsL) in the HLL flux) shares it;
+//   * the slope limiters in closed form (slope_limiter below).
+struct PerfectGasFast {
+    using real = double;
+    double gamma;
+    double gamma_m1;
+    double inv_gamma_m1;
+
+    __host__ __device__ static __forceinline__ double div(double a, double b)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        double y = __builtin_amdgcn_rcp(b);
+        double e = __builtin_fma(-b, y, 1.0);
+        y = __builtin_fma(y, e, y);
+        e = __builtin_fma(-b, y, 1.0);
+        y = __builtin_fma(y, e, y);
+        return a * y;
+#else
+        return a / b;
+#endif
+    }
+    __host__ __device__ static __forceinline__ double ratio(double a, double b) { return div(a, b); }
+    __host__ __device__ __forceinline__ double evol_from_pres(double, double P) const { return P * inv_gamma_m1; }
+    __host__ __device__ __forceinline__ double sound(double rho, double P) const { return sqrt(gamma * div(P, rho)); }
+    template <int ND>
+    __host__ __device__ __forceinline__ double internal_energy_fp64(const State<ND, double>& c) const
+    {
+        return c[ND + 1] - kinetic_cons_with(c, div);
+    }
+    template <int ND>
+    __host__ __device__ __forceinline__ double pressure_of_cons(const State<ND, double>& c) const
+    {
+        return gamma_m1 * internal_energy_fp64(c);
+    }
+    template <int ND>
+    __host__ __device__ __forceinline__ double pressure_of_cons(const State<ND, double>& c, double& evol) const
+    {
+        evol = internal_energy_fp64(c);
+        return gamma_m1 * evol;
+    }
+};
+
+// Policies whose state is FP64 (register budgets and kernel shapes follow the storage type).
+template <class Eos>
+constexpr bool kFp64State = std::is_same_v<typename Eos::real, double>;
+
 // euler_equations.hpp:41-49
 template <int ND, class T>
 __host__ __device__ __forceinline__ T kinetic_cons(const State<ND, T>& c)
@@ -322,6 +381,25 @@ template <int LIM>
 __host__ __device__ __forceinline__ double limited_slope_t(double dR, double dL)
 {
     return limited_slope_t<LIM, PerfectGas, double>(dR, dL);
+}
+
+// The limited slope a policy uses.  PerfectGasFast evaluates the limiters of slope_limiters.hpp
+// in closed form, with r = dR / dL and dL dR > 0:
+//   Minmod     2 min(1, r) / (1 + r) (dL + dR) / 2  = dR if |dR| < |dL|, else dL;
+//   VanLeer    (dR + dL) / 2 * 4 r / (r + 1)^2       = 2 dR dL / (dR + dL);
+//   VanAlbada  (dR + dL) / 2 * 2 r / (r^2 + 1)       = (dR + dL) dR dL / (dR^2 + dL^2);
+// the other policies follow the reference's expressions.
+template <int LIM, class Eos, class T>
+__host__ __device__ __forceinline__ T slope_limiter(T dR, T dL)
+{
+    if constexpr (std::is_same_v<Eos, PerfectGasFast> && LIM != kConstant) {
+        if (!(dL * dR > 0)) return 0;
+        if constexpr (LIM == kMinmod) return fabs(dR) < fabs(dL) ? dR : dL;
+        else if constexpr (LIM == kVanLeer) return Eos::div(2 * dR * dL, dR + dL);
+        else return Eos::div((dR + dL) * dR * dL, (dR * dR) + (dL * dL));
+    } else {
+        return limited_slope_t<LIM, Eos, T>(dR, dL);
+    }
 }
 
 // ---------------------------------------------------------------- Riemann solvers

@@ -2,7 +2,7 @@
 :354-531 (loop body) on one GPU.
 
 Two paths, both HIP:
-  * fused (default): one hclpp_step (hclpp_step_fx with one passive scalar)
+  * fused (default): one hclpp_step (hclpp_step_fx with passive scalars)
     per iteration -- the fused reconstruct/predict/Riemann/update kernel, the
     pressure fix when it is on, the ghost fill and the device clock (dt, t_end
     clipping, max_iter, the internal-energy check) -- with no host
@@ -10,8 +10,8 @@ Two paths, both HIP:
   * operators: the reference operator sequence (time_step,
     minimum_internal_energy, MUSCL-Hancock reconstruction, Godunov update,
     pressure fix, boundaries, cons->prim), one kernel per reference kernel,
-    host-synchronised on dt exactly like the reference.  Needed for more than
-    one passive scalar, and used to bisect parity.
+    host-synchronised on dt exactly like the reference.  Used to bisect
+    parity and for user-supplied strategies.
 With an output schedule (simulate(), io.OutputSchedule) dt is clipped to the
 output times and outputs are written in the reference's HDF5 layout, as
 main.cpp:313-345 and :357-375, :532-557 do; restart() reads such a file back
@@ -42,26 +42,32 @@ def _dev(a, shape, device="cuda"):
 class HydroSolver:
     def __init__(self, grid: Grid, eos: PerfectGas, *, reconstruction: str = "VanLeer", riemann_solver: str = "HLL",
                  gravity=None, bc=None, bc_order=None, cfl: float = 0.4, nfx: int = 0, fused: bool = True,
-                 precision: str = "fp64", pressure_fix: str = "Off", eps_pf: float = 1e-6):
+                 precision: str = "fp64", pressure_fix: str = "Off", eps_pf: float = 1e-6, arithmetic: str = "reference"):
         if precision not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32' (FP32 storage with FP64 pressure recovery)")
         if pressure_fix not in ("On", "Off"):
             raise ValueError("pressure_fix must be 'On' or 'Off'")
         self.pressure_fix = pressure_fix == "On"  # main.cpp:459
         self.eps_pf = float(eps_pf)
-        if precision == "fp32" and (not fused or nfx > 1 or self.pressure_fix):
-            raise ValueError("FP32 storage is a fused-step mode (at most one passive scalar, no pressure fix)")
+        if precision == "fp32" and (not fused or self.pressure_fix):
+            raise ValueError("FP32 storage is a fused-step mode (no pressure fix)")
+        if arithmetic not in ("reference", "fast"):
+            raise ValueError("arithmetic must be 'reference' or 'fast'")
+        if arithmetic == "fast" and (not fused or precision != "fp64" or nfx > 0 or self.pressure_fix):
+            raise ValueError("the fast FP64 arithmetic is a fused FP64 mode (no passive scalars, no pressure fix)")
+        self.arithmetic = arithmetic
         self.grid = grid
         self.eos = eos
         self.gravity = gravity
         self.cfl = float(cfl)
         self.nfx = nfx
-        # the fused step carries at most one passive scalar; the operator path takes any nfx
-        self.fused = fused and nfx <= 1
+        self.fused = fused
         self.precision = precision
         self.ctx = Context(grid, eos, gravity, bc, bc_order, nfx, reconstruction, riemann_solver, precision)
         if self.fused and self.pressure_fix:  # the fused step recomputes the face states the fix reads
             check(lib().hclpp_set_pressure_fix(self.ctx.handle, 1, self.eps_pf))
+        if arithmetic == "fast":
+            check(lib().hclpp_set_fast_arithmetic(self.ctx.handle, 1))
         nd = grid.ndim
         z = grid.zeros
         self._cons = [(z(), z(nd), z()), (z(), z(nd), z())]

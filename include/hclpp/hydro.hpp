@@ -626,7 +626,7 @@ public:
 };
 
 // ------------------------------------------------------------ main.cpp:305-311, 354-531 (fused path)
-// Owns a double-buffered conservative state (and passive scalars, nfx <= 1) and
+// Owns a double-buffered conservative state (and passive scalars) and
 // runs fused steps with the device clock: no host synchronisation inside the loop.
 template <class Gravity>
 class HydroDriver
@@ -648,7 +648,7 @@ public:
                 DistributedBoundaryCondition const& bcs, double cfl, int nfx = 0)
         : m_grid(grid), m_cfl(cfl), m_nfx(nfx)
     {
-        if (nfx < 0 || nfx > 1) throw std::runtime_error("the fused driver carries at most one passive scalar (use the operators)");
+        if (nfx < 0) throw std::runtime_error("nfx must be >= 0");
         int const lim = hclpp_limiter_from_name(reconstruction.c_str());
         if (lim < 0) throw std::runtime_error("Unknown face reconstruction algorithm: " + reconstruction + ".");
         int const rs = hclpp_riemann_from_name(riemann.c_str());

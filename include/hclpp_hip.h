@@ -201,6 +201,17 @@ typedef struct {
  * storage only; on = 0 turns it off.  The reverse step refuses a context with the fix on. */
 int hclpp_set_pressure_fix(hclpp_ctx* ctx, int on, double eps);
 
+/* Opt-in fast FP64 arithmetic for the fused step (on = 1; on = 0 restores the reference
+ * arithmetic).  No counterpart in the reference, whose arithmetic hclpp_step reproduces bit for
+ * bit; this mode keeps the FP64 state and every formula of the step but not their rounding:
+ * P = (gamma-1) e without the temperature round trip through kb and mp (perfect_gas.hpp:46-78),
+ * divisions as a product with a shared reciprocal, the slope limiters (slope_limiters.hpp:36-86)
+ * in closed form, and fused multiply-adds.  Its results drift from the reference's by rounding,
+ * growing with the step count.  Affects hclpp_step and hclpp_step_dt on an FP64 context; the
+ * operator entry points keep the reference arithmetic.  HCLPP_ERR_UNSUPPORTED with passive
+ * scalars (nfx > 0) or the pressure fix on. */
+int hclpp_set_fast_arithmetic(hclpp_ctx* ctx, int on);
+
 /* Output schedule of main.cpp:362-375 (utils/param.cpp:44-47), applied from the next
  * hclpp_clock_start on: iteration outputs every iter_output_frequency steps (0: none), time
  * outputs at time_first_output + k time_output_frequency (frequency 0: none), and the output
@@ -228,13 +239,14 @@ int hclpp_clock_read(hclpp_ctx* ctx, hclpp_clock* out);
  * FP64 storage; HCLPP_ERR_UNSUPPORTED when the context has nfx > 0 (see hclpp_step_fx). */
 int hclpp_step(hclpp_ctx* ctx, const double* rho, const double* rhou, const double* E, double* rho_new, double* rhou_new,
                double* E_new);
-/* hclpp_step carrying the context's passive scalar (nfx = 1; fx and fx_new hold nfx arrays of
+/* hclpp_step carrying the context's passive scalars (nfx >= 1; fx and fx_new hold nfx arrays of
  * mass fractions, layout [f][cell], ghosts filled on input as hclpp_fill_ghosts leaves them):
- * the main loop body of main.cpp:426-457 with nfx > 0, i.e. the scalar reconstructed with the
+ * the main loop body of main.cpp:426-457 with nfx > 0, i.e. each scalar reconstructed with the
  * state (hydro_reconstruction.hpp:99-109), carried through the Hancock half step as a partial
  * density (extrapolation_reconstruction.hpp:87-88, 105-108, 315-337), upwinded with the mass
- * flux and clamped to [0, 1] in the update (godunov_scheme.hpp:122, 263-301).  nfx > 1:
- * HCLPP_ERR_UNSUPPORTED (the operator entry points take any nfx). */
+ * flux and clamped to [0, 1] in the update (godunov_scheme.hpp:122, 263-301).  The fused kernel
+ * carries one scalar per launch: nfx > 1 launches it nfx times (the first also writes the
+ * hydro update), so the hydro arithmetic is repeated per scalar. */
 int hclpp_step_fx(hclpp_ctx* ctx, const double* rho, const double* rhou, const double* E, const double* fx, double* rho_new,
                   double* rhou_new, double* E_new, double* fx_new);
 

@@ -1,7 +1,8 @@
 """The GPU step against the analytic solutions of the reference's own test problems
 (test/exact_shock_tube.py, test/exact_sedov.py), independent of the oracle:
 
-* Sod (1000 cells, Minmod + HLL, CFL 0.4, t = 0.2): the fused step, FP64 and FP32 storage,
+* Sod (1000 cells, Minmod + HLL, CFL 0.4, t = 0.2): the fused step, FP64 (reference and fast
+  arithmetic) and FP32 storage,
   within L1(rho) < 2e-3 of the exact Riemann solution (Toro ch. 4; star state pinned against
   Toro's table in the oracle tests);
 * 3D spherical Sedov (48 x 8 x 16, HLL + VanLeer, E = 1 in the two innermost shells, rho = 1):
@@ -20,10 +21,14 @@ from tests.helpers import make_solver
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp64_fast"])
 def test_sod_against_exact_solution(cuda, precision):
+    from heraclespp_amd import _lib
+
     s = setups.sod(1000)
-    sol = make_solver(s, precision=precision)
+    sol = make_solver(s, precision=precision.replace("_fast", ""))
+    if precision == "fp64_fast":  # the opt-in fast FP64 arithmetic (hclpp_set_fast_arithmetic)
+        _lib.check(_lib.lib().hclpp_set_fast_arithmetic(sol.ctx.handle, 1))
     sol.init_from_prim(s.rho, s.u, s.P)
     sol.run(s.t_end, 1 << 30)
     assert abs(sol.t - 0.2) < 1e-15

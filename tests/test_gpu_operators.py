@@ -243,24 +243,20 @@ def test_boundaries(cuda, nd, bcs, order):
 
 
 def test_fused_step_passive_scalar_rules(cuda):
-    """The fused step carries one passive scalar (hclpp_step_fx); the C-ABI refuses, before
-    touching any array, hclpp_step on a context with nfx > 0 and hclpp_step_fx with nfx > 1
-    (the solver routes nfx > 1 to the operator path)."""
+    """The fused step carries the passive scalars through hclpp_step_fx (any nfx); the C-ABI
+    refuses, before touching any array, hclpp_step on a context with nfx > 0."""
     import ctypes
 
     from heraclespp_amd import _lib, setups
     from tests.helpers import make_solver
 
     s = setups.kelvin_helmholtz(16)
-    one = make_solver(s, fused=True, nfx=1)
-    assert one.fused
     fake = [ctypes.c_void_p(64 * (k + 1)) for k in range(8)]
-    with pytest.raises(_lib.HclppError, match="hclpp_step_fx"):
-        _lib.check(_lib.lib().hclpp_step(one.ctx.handle, *fake[:6]))
-    two = make_solver(s, fused=True, nfx=2)
-    assert not two.fused
-    with pytest.raises(_lib.HclppError, match="at most one passive scalar"):
-        _lib.check(_lib.lib().hclpp_step_fx(two.ctx.handle, *fake))
+    for nfx in (1, 2):
+        sol = make_solver(s, fused=True, nfx=nfx)
+        assert sol.fused
+        with pytest.raises(_lib.HclppError, match="hclpp_step_fx"):
+            _lib.check(_lib.lib().hclpp_step(sol.ctx.handle, *fake[:6]))
 
 
 @pytest.mark.parametrize("fused", [True, False])
