@@ -25,7 +25,7 @@ from dataclasses import dataclass, field
 import torch
 
 from ._lib import check, lib
-from .hydro import Grid, PerfectGas, UniformGravity
+from .hydro import Context, Grid, PerfectGas, UniformGravity
 
 
 def _ptr(t):
@@ -47,12 +47,25 @@ class RolloutSpec:
     gravity: UniformGravity | None = None
     mmw: float = 1.0
     _engines: dict = field(default_factory=dict, repr=False)
+    _side_engines: dict = field(default_factory=dict, repr=False)
 
     def engine(self, gamma: float) -> "StepEngine":
         e = self._engines.get(gamma)
         if e is None:
             e = StepEngine(self, gamma)
             self._engines[gamma] = e
+        return e
+
+    def side_engine(self, gamma: float) -> "StepEngine":
+        """An engine whose context runs on a stream of its own (default priority), for the segment
+        recomputation that overlaps the reverse steps."""
+        e = self._side_engines.get(gamma)
+        if e is None:
+            stream = torch.cuda.Stream()
+            with torch.cuda.stream(stream):
+                e = StepEngine(self, gamma, own_context=True)
+            e.stream = stream
+            self._side_engines[gamma] = e
         return e
 
     @property
@@ -65,11 +78,13 @@ class RolloutSpec:
 class StepEngine:
     """One context: fixed-dt forward steps (hclpp_step_dt) and reverse steps (hclpp_step_adjoint)."""
 
-    def __init__(self, spec: RolloutSpec, gamma: float):
+    def __init__(self, spec: RolloutSpec, gamma: float, own_context: bool = False):
+        """own_context: a context of its own (its own stream, clock and scratch) instead of the
+        grid's shared one for this configuration."""
         self.spec = spec
         self.gamma = gamma
-        self.ctx = spec.grid.context(PerfectGas(gamma, spec.mmw), spec.gravity, spec.bc, None, 0, spec.reconstruction,
-                                     spec.riemann_solver, "fp64")
+        args = (PerfectGas(gamma, spec.mmw), spec.gravity, spec.bc, None, 0, spec.reconstruction, spec.riemann_solver, "fp64")
+        self.ctx = Context(spec.grid, *args) if own_context else spec.grid.context(*args)
         self.nd = spec.grid.ndim
 
     def new_state(self):
@@ -190,7 +205,7 @@ def binomial_split(steps: int, snaps: int) -> int:
 
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, rho, rhou, E, gamma, spec, dts, every, schedule, snapshots):
+    def forward(ctx, rho, rhou, E, gamma, spec, dts, every, schedule, snapshots, overlap):
         eng = spec.engine(float(gamma))
         U0 = eng.load(rho, rhou, E)
         N = len(dts)
@@ -200,6 +215,7 @@ class _Rollout(torch.autograd.Function):
             keep = {0}
         kept, last = forward_states(eng, U0, dts, keep)
         ctx.spec, ctx.dts, ctx.every, ctx.schedule, ctx.snapshots = spec, list(dts), every, schedule, snapshots
+        ctx.overlap = overlap
         ctx.kept = kept
         ctx.gamma_shape, ctx.gamma_device = gamma.shape, gamma.device
         ctx.gamma_value = float(gamma)
@@ -229,7 +245,13 @@ class _Rollout(torch.autograd.Function):
 
         N = len(ctx.dts)
         eng.start(ctx.kept[0])  # the clock only gates the recomputation steps here; dt comes from the schedule
-        if ctx.schedule == "periodic":
+        if ctx.schedule == "periodic" and ctx.overlap and len(ctx.kept) > 1:
+            side = spec.side_engine(ctx.gamma_value)
+            side.start(ctx.kept[0])
+            _reverse_periodic_overlapped(eng, side, ctx.kept, ctx.dts, ctx.every, dgam, state)
+            side.check_clock(state["forward_steps"])
+            state["forward_steps"] = 0  # the main engine's clock counted none
+        elif ctx.schedule == "periodic":
             _reverse_periodic(eng, ctx.kept, ctx.dts, ctx.every, reverse, state, batched)
         else:
             _reverse_binomial(eng, ctx.kept[0], 0, N, ctx.dts, ctx.snapshots, reverse, state)
@@ -237,8 +259,8 @@ class _Rollout(torch.autograd.Function):
         eng.check_clock(state["forward_steps"])
         lam = state["lam"]
         out = (lam[0][it].clone(), lam[1][(slice(None),) + it].clone(), lam[2][it].clone())
-        _Rollout.last_backward = {"reverse_steps": state["reverse_steps"], "recomputed_steps": state["forward_steps"]}
-        return out + (dgam.reshape(ctx.gamma_shape).to(ctx.gamma_device), None, None, None, None, None)
+        _Rollout.last_backward = {"reverse_steps": state["reverse_steps"], "recomputed_steps": state.get("recomputed", state["forward_steps"])}
+        return out + (dgam.reshape(ctx.gamma_shape).to(ctx.gamma_device), None, None, None, None, None, None)
 
 
 def _reverse_segment(eng, seg, s, e, dts, dgam, state):
@@ -275,6 +297,46 @@ def _reverse_periodic(eng, kept, dts, every, reverse, state, batched=None):
                 reverse(seg[n - s], n)
 
 
+def _reverse_periodic_overlapped(eng, side, kept, dts, every, dgam, state):
+    """_reverse_periodic with the recomputation of the next segment (on the side engine's stream)
+    running while the current one is reversed (on the main engine's stream, the caller's current
+    stream).  Two pools of segment states alternate; events order each recomputation after the
+    reversal that last read its pool, and each reversal after its segment's recomputation.  The
+    kernels and their order on each stream are those of _reverse_periodic, so the gradients are
+    bit-identical to it."""
+    N = len(dts)
+    starts = sorted(kept, reverse=True)
+    pools = [[eng.new_state() for _ in range(min(every, N))] for _ in range(2)]
+    ready = [torch.cuda.Event(), torch.cuda.Event()]
+    free = [None, None]
+    main = torch.cuda.current_stream()
+    segs = [None, None]
+
+    def recompute(i, s):
+        e = min(s + every, N)
+        seg = [kept[s]] + pools[i][:e - 1 - s]
+        with torch.cuda.stream(side.stream):
+            side.stream.wait_stream(main) if free[i] is None else side.stream.wait_event(free[i])
+            side.steps(seg, dts[s:e - 1])
+            ready[i].record(side.stream)
+        state["recomputed"] = state.get("recomputed", 0) + e - 1 - s
+        segs[i] = (seg, s, e)
+
+    recompute(0, starts[0])
+    for idx in range(len(starts)):
+        i = idx % 2
+        if idx + 1 < len(starts):
+            recompute(1 - i, starts[idx + 1])
+        seg, s, e = segs[i]
+        main.wait_event(ready[i])
+        _reverse_segment(eng, seg, s, e, dts, dgam, state)
+        ev = torch.cuda.Event()
+        ev.record(main)
+        free[i] = ev
+    main.wait_stream(side.stream)
+    state["forward_steps"] = state["recomputed"]
+
+
 def _reverse_binomial(eng, U, first, last, dts, snaps, reverse, state):
     """Reverse steps [first, last) starting from state U at step `first`, holding at most `snaps` extra states."""
     if last - first == 1:
@@ -303,19 +365,20 @@ def _reverse_binomial(eng, U, first, last, dts, snaps, reverse, state):
 
 
 def rollout(rho, rhou, E, gamma, spec: RolloutSpec, dts, checkpoint_every: int = 20, schedule: str = "periodic",
-            snapshots: int = 8):
+            snapshots: int = 8, overlap: bool = True):
     """Differentiable N-step rollout of interior arrays (rho, rhou, E) with the fixed dt schedule `dts`.
 
     gamma: a 0-d float64 tensor (may require grad).  schedule: "periodic" (a checkpoint every
     `checkpoint_every` steps, segment recomputation) or "binomial" (at most `snapshots` live
-    states, revolve placement)."""
+    states, revolve placement).  overlap (periodic schedule): recompute the next segment on a
+    second stream while the current one is reversed (same results, bit for bit)."""
     if schedule not in ("periodic", "binomial"):
         raise ValueError("schedule must be 'periodic' or 'binomial'")
     if checkpoint_every < 1:
         raise ValueError("checkpoint_every must be >= 1")
     if not torch.is_tensor(gamma):
         gamma = torch.tensor(float(gamma), dtype=torch.float64)
-    return _Rollout.apply(rho, rhou, E, gamma, spec, list(dts), checkpoint_every, schedule, snapshots)
+    return _Rollout.apply(rho, rhou, E, gamma, spec, list(dts), checkpoint_every, schedule, snapshots, overlap)
 
 
 class DiffEulerProblem:

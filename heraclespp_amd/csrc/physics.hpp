@@ -9,8 +9,9 @@
 //    the reference operation order and rounds like it (built with
 //    -ffp-contract=off), so results are bit-identical to the reference;
 //  * PerfectGasMixed (FP32): state and fluxes in FP32 with hardware reciprocal
-//    divisions, while the pressure recovery P = (gamma-1)(E - |m|^2 / 2 rho),
-//    which cancels at high Mach number, is evaluated in FP64;
+//    divisions and closed-form slope limiters, while the pressure recovery
+//    P = (gamma-1)(E - |m|^2 / 2 rho), which cancels at high Mach number, is
+//    evaluated in FP64;
 //  * PerfectGasFast (FP64, opt-in): the same FP64 state and algorithm without
 //    the reference's rounding -- P = (gamma-1) e directly instead of the
 //    temperature round trip through kb and mp, divisions as a product with a
@@ -383,16 +384,17 @@ __host__ __device__ __forceinline__ double limited_slope_t(double dR, double dL)
     return limited_slope_t<LIM, PerfectGas, double>(dR, dL);
 }
 
-// The limited slope a policy uses.  PerfectGasFast evaluates the limiters of slope_limiters.hpp
-// in closed form, with r = dR / dL and dL dR > 0:
+// The limited slope a policy uses.  PerfectGasFast and PerfectGasMixed (FP32: one reciprocal
+// instead of two or three; measured +5 % on RT 512^3) evaluate the limiters of
+// slope_limiters.hpp in closed form, with r = dR / dL and dL dR > 0:
 //   Minmod     2 min(1, r) / (1 + r) (dL + dR) / 2  = dR if |dR| < |dL|, else dL;
 //   VanLeer    (dR + dL) / 2 * 4 r / (r + 1)^2       = 2 dR dL / (dR + dL);
 //   VanAlbada  (dR + dL) / 2 * 2 r / (r^2 + 1)       = (dR + dL) dR dL / (dR^2 + dL^2);
-// the other policies follow the reference's expressions.
+// PerfectGas follows the reference's expressions.
 template <int LIM, class Eos, class T>
 __host__ __device__ __forceinline__ T slope_limiter(T dR, T dL)
 {
-    if constexpr (std::is_same_v<Eos, PerfectGasFast> && LIM != kConstant) {
+    if constexpr ((std::is_same_v<Eos, PerfectGasFast> || std::is_same_v<Eos, PerfectGasMixed>) && LIM != kConstant) {
         if (!(dL * dR > 0)) return 0;
         if constexpr (LIM == kMinmod) return fabs(dR) < fabs(dL) ? dR : dL;
         else if constexpr (LIM == kVanLeer) return Eos::div(2 * dR * dL, dR + dL);

@@ -352,9 +352,9 @@ static void test_pressure_fix_loop()
 // main.cpp:305-531 with one passive scalar (nfx = 1, a tracer of the dense band): the reference
 // operator sequence through the interface classes and the fused driver (hclpp_step_fx), both
 // against the oracle driver; every array, the scalar included, must be bit-identical.
-static void test_passive_scalar_loop()
+static void test_passive_scalar_loop(int const nfx)
 {
-    int const nd = 2, n = 32, nfx = 1, nsteps = 10;
+    int const nd = 2, n = 32, nsteps = 10;
     double const cfl = 0.4, gamma = 1.4;
     Grid const grid(nd, {n, n, 1}, 2, Geometry::Geom_cartesian, {0, 0, 0}, {1, 1, 1});
     PerfectGas const eos(gamma, 1.);
@@ -371,7 +371,10 @@ static void test_passive_scalar_loop()
             hu[c] = band ? 0.5 : -0.5;
             hu[c + cells] = 0.01 * std::sin(4 * M_PI * x) * std::sin(5.0 * x + 2.0 * y);
             hP[c] = 2.5;
-            hfx[c] = band ? 1.0 : 0.0;
+            // scalar 0 traces the band, 1 its complement, 2 a smooth field (nfx > 1: one fused
+            // launch per scalar)
+            for (int f = 0; f < nfx; ++f)
+                hfx[c + cells * f] = f == 0 ? (band ? 1.0 : 0.0) : f == 1 ? (band ? 0.0 : 1.0) : 0.5 + 0.4 * std::sin(2 * M_PI * (x + f * y));
         }
     or_config oc {};
     oc.ndim = nd;
@@ -596,7 +599,8 @@ int main()
         test_operator_loop();
         test_internal_mass_loop();
         test_pressure_fix_loop();
-        test_passive_scalar_loop();
+        test_passive_scalar_loop(1);
+        test_passive_scalar_loop(3);
         test_user_face_reconstruction();
     } catch (std::exception const& e) {
         std::printf("FAIL: exception %s\n", e.what());

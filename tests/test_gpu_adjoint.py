@@ -124,9 +124,10 @@ def _cons0(rho0, u, P, gamma):
     return rho0, rhou, E
 
 
-def _loss(spec, rho0, u, P, gamma, dts, target, every=8, schedule="periodic", snapshots=4, target_E=None):
+def _loss(spec, rho0, u, P, gamma, dts, target, every=8, schedule="periodic", snapshots=4, target_E=None, overlap=True):
     r, m, E = _cons0(rho0, u, P, gamma)
-    rT, _, ET = rollout(r, m, E, gamma, spec, dts, checkpoint_every=every, schedule=schedule, snapshots=snapshots)
+    rT, _, ET = rollout(r, m, E, gamma, spec, dts, checkpoint_every=every, schedule=schedule, snapshots=snapshots,
+                        overlap=overlap)
     dV = 1.0 / rho0.numel()
     L = 0.5 * torch.sum((rT - target) ** 2) * dV
     if target_E is not None:
@@ -214,10 +215,13 @@ def test_checkpoint_schedules_agree_bitwise(cuda):
     dts = cfl_schedule(spec, r, m, E, 1.4, 23, s.cfl)
     target = torch.ones_like(rho0)
     grads = []
-    for schedule, every, snaps in (("periodic", 5, 0), ("periodic", 23, 0), ("binomial", 1, 3), ("binomial", 1, 1)):
+    # periodic with the next segment recomputed on a second stream during each reversal (overlap),
+    # and without; single-segment periodic; binomial with 3 and 1 snapshots
+    for schedule, every, snaps, overlap in (("periodic", 5, 0, True), ("periodic", 5, 0, False), ("periodic", 4, 0, True),
+                                            ("periodic", 23, 0, True), ("binomial", 1, 3, False), ("binomial", 1, 1, False)):
         x = rho0.clone().requires_grad_(True)
         gamma = gamma0.clone().requires_grad_(True)
-        L, _ = _loss(spec, x, u, P, gamma, dts, target, every=every, schedule=schedule, snapshots=snaps)
+        L, _ = _loss(spec, x, u, P, gamma, dts, target, every=every, schedule=schedule, snapshots=snaps, overlap=overlap)
         L.backward()
         grads.append((x.grad.clone(), gamma.grad.clone()))
     for g in grads[1:]:

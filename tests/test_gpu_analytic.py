@@ -8,7 +8,12 @@
 * 3D spherical Sedov (48 x 8 x 16, HLL + VanLeer, E = 1 in the two innermost shells, rho = 1):
   the radius of the shell-averaged density peak follows the Sedov-Taylor law
   R = (E t^2 / (alpha rho))^(1/5), alpha(gamma = 1.4) = 0.851072 (Kamm & Timmes 2007), within
-  one radial cell, after 20000 steps."""
+  one radial cell, after 20000 steps, and at the configuration's t_end = 1.0 on 96 x 8 x 16
+  (about 148 000 steps: the CFL step is set by the narrowest cells around the inner radius, ghost
+  cells included): the shock front (the outermost cell above half the peak
+  density) within one radial cell of R(1) = 1.033, and the density maximum at most two cells
+  behind it.  On the configured 512 x 128 x 256 grid the cells beside the poles at the inner
+  radius are some 250 times narrower and t_end would take about 2e7 steps."""
 
 import numpy as np
 import pytest
@@ -55,3 +60,25 @@ def test_sedov_shock_radius(cuda):
     dr = x[3] - x[2]
     print(f"\n[analytic] sedov t={sol.t:.4f}: peak radius {r_peak:.4f}, Sedov-Taylor {r_st:.4f}, cell {dr:.4f}", flush=True)
     assert abs(r_peak - r_st) <= dr
+
+
+def test_sedov_configured_t_end(cuda):
+    s = setups.sedov(96, 8, 16, t_end=1.0)
+    sol = make_solver(s)
+    sol.init_from_prim(s.rho, s.u, s.P)
+    steps = sol.run(s.t_end, 1 << 30)
+    assert sol.t == 1.0
+    rho = sol.rho.cpu().numpy()
+    profile = rho[2:-2, 2:-2, 2:-2].mean(axis=(0, 1))
+    x = orc.regular_nodes(96, 2, s.xmin[0], s.xmax[0])
+    xc = ((x[:-1] + x[1:]) / 2)[2:-2]
+    k = int(np.argmax(profile))
+    r_peak = xc[k]
+    # the front: the outermost cell whose density is above halfway between the ambient value and the peak
+    r_front = xc[k + int(np.nonzero(profile[k:] > 1.0 + 0.5 * (profile[k] - 1.0))[0][-1])]
+    r_st = (1.0 * 1.0 ** 2 / (0.851072 * 1.0)) ** 0.2
+    dr = x[3] - x[2]
+    print(f"\n[analytic] sedov 96x8x16 t=1.0 after {steps} steps: front {r_front:.4f}, peak {r_peak:.4f}, "
+          f"Sedov-Taylor {r_st:.4f}, cell {dr:.4f}", flush=True)
+    assert abs(r_front - r_st) <= dr
+    assert 0 <= r_st - r_peak <= 2 * dr  # a captured shock's density maximum trails the front by a cell or two
