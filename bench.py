@@ -226,7 +226,7 @@ def calibrate_copy(nbytes: int):
 
 
 def _timed_run(workload: str, precision: str, steps: int, warmup: int, warmup_seconds: float, total: int,
-               arithmetic: str = "reference"):
+               arithmetic: str = "reference", eps_pf: float | None = None):
     """Fused steps of a workload with the state resident in HBM: `warmup` untimed steps, more until
     `warmup_seconds` have passed (at most `total` - `steps` in all), then exactly `steps` timed steps
     between synchronisations, then untimed steps up to `total` (if more).  Returns a record: the setup
@@ -245,7 +245,8 @@ def _timed_run(workload: str, precision: str, steps: int, warmup: int, warmup_se
     grid = Grid(s.ndim, s.n, s.ng, s.geom, s.xmin, s.xmax)
     solver = HydroSolver(grid, PerfectGas(s.gamma, 1.0), reconstruction=s.reconstruction,
                          riemann_solver=s.riemann_solver, gravity=UniformGravity(s.gravity), bc=s.bc, cfl=s.cfl,
-                         precision=precision, arithmetic=arithmetic)
+                         precision=precision, arithmetic=arithmetic,
+                         pressure_fix="Off" if eps_pf is None else "On", eps_pf=1e-6 if eps_pf is None else eps_pf)
     solver.init_from_prim(s.rho, s.u, s.P)
     del s.rho, s.u, s.P
     cells = 1
@@ -412,6 +413,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-storage run of the same workload")
     ap.add_argument("--no-kh", action="store_true", help="skip the 2D Kelvin-Helmholtz 2048^2 FP64 / FP32 block")
     ap.add_argument("--no-fast", action="store_true", help="skip the fast-FP64-arithmetic block")
+    ap.add_argument("--pressure-fix", type=float, default=None, metavar="EPS",
+                    help="run the headline workload with the pressure fix on (eps_pf = EPS; FP64, fused)")
     ap.add_argument("--arithmetic", default="reference", choices=["reference", "fast"],
                     help="fast: the headline run with the opt-in fast FP64 arithmetic (not bit-identical to the reference)")
     ap.add_argument("--no-accuracy", action="store_true",
@@ -434,8 +437,12 @@ def main():
     torch.cuda.set_device(0)
     if a.arithmetic == "fast" and a.precision != "fp64":
         ap.error("--arithmetic fast is an FP64 mode")
-    total = 0 if (a.no_accuracy or (a.no_mixed and a.no_fast) or a.precision != "fp64") else WORKLOADS[a.workload][3]
-    r = _timed_run(a.workload, a.precision, a.steps, a.warmup, a.warmup_seconds, total, arithmetic=a.arithmetic)
+    if a.pressure_fix is not None and (a.precision != "fp64" or a.arithmetic != "reference"):
+        ap.error("--pressure-fix runs with FP64 storage and the reference arithmetic")
+    total = 0 if (a.no_accuracy or (a.no_mixed and a.no_fast) or a.precision != "fp64" or a.arithmetic != "reference"
+                  or a.pressure_fix is not None) else WORKLOADS[a.workload][3]
+    r = _timed_run(a.workload, a.precision, a.steps, a.warmup, a.warmup_seconds, total, arithmetic=a.arithmetic,
+                   eps_pf=a.pressure_fix)
     s, cells = r.setup, r.cells
     wall = r.wall
     value = r.value
@@ -481,6 +488,7 @@ def main():
             "riemann_solver": s.riemann_solver,
             "storage": a.precision,
             "arithmetic": a.arithmetic,
+            "pressure_fix": None if a.pressure_fix is None else {"eps_pf": a.pressure_fix},
             "path": "fused (hclpp_step: march kernel + ghost fill + device dt)",
             "launch": r.plan,
             "warmup_steps_run": r.warmup_steps,
@@ -507,7 +515,7 @@ def main():
     }
     rho64, total = r.rho, r.total
     del r
-    reference_fp64 = a.precision == "fp64" and a.arithmetic == "reference"
+    reference_fp64 = a.precision == "fp64" and a.arithmetic == "reference" and a.pressure_fix is None
     if reference_fp64 and not a.no_mixed:
         line["mixed_fp32"] = mixed_block(a.workload, a, value, rho64, total)
         torch.cuda.empty_cache()
