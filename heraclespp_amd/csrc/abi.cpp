@@ -63,7 +63,7 @@ struct hclpp_ctx {
     // pressure fix of the fused step (hclpp_set_pressure_fix)
     int pf_on = 0;
     double pf_eps = 1e-6;
-    double* fix_partials = nullptr;  // two doubles per workgroup of the interior range
+    double* fix_partials = nullptr;  // two doubles per workgroup of the fused (march) kernel
     int64_t fix_nb = 0;
     // output schedule applied by the next clock start (hclpp_clock_set_outputs)
     int out_iter_freq = 0;
@@ -718,7 +718,7 @@ static int fused_step(hclpp_ctx* c, const S* rho, const S* rhou, const S* E, con
     }
     if constexpr (!f32) {
         if (c->pf_on) {  // main.cpp:459-475: after the update, before the boundaries
-            launch_pressure_fix_fused(nd, c->cfg.geom, c->cfg.limiter, c->G, eos, A, interior_range(c->G), c->fix_partials, c->stream);
+            launch_pressure_fix_fused(nd, c->cfg.geom, c->cfg.limiter, c->G, eos, A, c->march, c->fix_partials, c->stream);
             HCLPP_LAUNCHED("pressure fix (fused step)");
         }
     }
@@ -765,8 +765,13 @@ int hclpp_set_pressure_fix(hclpp_ctx* c, int on, double eps)
     if (on && c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "the pressure fix runs with FP64 storage only");
     if (on && c->fast_on) return fail(HCLPP_ERR_UNSUPPORTED, "the pressure fix runs with the reference arithmetic only");
     if (on && !std::isfinite(eps)) return fail(HCLPP_ERR_ARG, "eps must be finite");
-    if (on && !c->fix_partials) {
-        int64_t const nb = reduce_blocks(interior_range(c->G));
+    if (on && (!c->fix_partials || c->fix_nb != c->fused_nb)) {  // one pair per workgroup of the march kernel's plan
+        if (c->fix_partials) {
+            HCLPP_HIP(hipStreamSynchronize(c->stream));
+            HCLPP_HIP(hipFree(c->fix_partials));
+            c->fix_partials = nullptr;
+        }
+        int64_t const nb = c->fused_nb;
         HCLPP_HIP(hipMalloc(&c->fix_partials, sizeof(double) * 2 * (nb > 0 ? nb : 1)));
         c->fix_nb = nb;
     }

@@ -25,7 +25,8 @@ struct DevClock {
 };
 
 // Per-workgroup partials of the fused step: [0] max CFL rate, [1] min internal
-// energy, [2..2+NV) sums of the conserved variables times dV.
+// energy, [2..2+NV) sums of the conserved variables times dV, [kPartialStride - 1] 1 when the
+// pressure fix rewrites some cell of the workgroup (else 0).
 constexpr int kPartialStride = 8;
 
 // Arrays of one fused step; S is the storage type (double, or float for FP32 storage).
@@ -112,8 +113,9 @@ void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, c
 void launch_fused_step_fx(int nd, int geom, int rs, int lim, const GridDev& G, const MarchPlan& P, const PerfectGasMixed& eos,
                           const StepArraysF& A, hipStream_t s);
 int march_occupancy_fx(int nd, int geom, int rs, int lim, bool fp32);
-// the fused step's pressure fix (pressure_fix_fused.hip): after k_step_march, over the interior r
-void launch_pressure_fix_fused(int nd, int geom, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, const Range3& r,
+// the fused step's pressure fix (pressure_fix_fused.hip): after k_step_march, over the march
+// kernel's workgroups (plan P); fix_partials receives two doubles per workgroup
+void launch_pressure_fix_fused(int nd, int geom, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, const MarchPlan& P,
                                double* fix_partials, hipStream_t s);
 GhostShell make_ghost_shell(const GridDev& G, int nd);
 int64_t ghost_blocks(const GhostShell& B);

@@ -329,6 +329,70 @@ __device__ __forceinline__ State<ND, T> dummy_state()
     return p;
 }
 
+// The columns of a march workgroup.  Workgroup b of nb owns a tile of columns (3D: HX x HY, 2D: a
+// row of HX, 1D: NT segments of one column) and a segment of planes of the march dim; thread tid
+// owns column (a, bb) (ghost-inclusive coordinates of the in-plane dims; a tile's first and last
+// columns are halo) and planes [zs, ze).  Blocks that share an XCD (b % 8) get consecutive tiles,
+// so halo re-reads hit that XCD's L2.
+struct MarchColumn {
+    int hx, hy, ox, oy, a, bb, zs, ze;
+    bool in_block;      // the thread owns a column of the tile
+    bool col_in_array;  // the column exists in memory
+    bool hancock_ok;    // the column needs a predictor
+    bool out_col;       // the column holds output cells
+    bool seg_ok;        // the segment holds planes
+};
+
+template <int ND, int HX, int HY, int NT>
+__device__ __forceinline__ MarchColumn march_column(const GridDev& G, const FusedLayout& L, int b, int nb, int tid)
+{
+    constexpr int MD = ND - 1;
+    constexpr int TX = (ND >= 2) ? HX - 2 : 1;
+    constexpr int TY = (ND == 3) ? HY - 2 : 1;
+    MarchColumn c{};
+    int const q = nb / 8, rem = nb % 8, x8 = b % 8;
+    int const lb = x8 * q + (x8 < rem ? x8 : rem) + b / 8;
+    int seg;
+    if constexpr (ND == 1) {
+        seg = lb * NT + tid;
+    } else {
+        c.hx = tid % HX;
+        c.hy = tid / HX;
+        int const tx = lb % L.ntile[0];
+        int const rest = lb / L.ntile[0];
+        int const ty = rest % L.ntile[1];
+        seg = rest / L.ntile[1];
+        c.ox = G.ngh[0] + tx * TX;
+        c.oy = (ND == 3) ? G.ngh[1] + ty * TY : 0;
+    }
+    c.in_block = (ND == 1) || tid < HX * HY;
+    c.a = c.ox - 1 + c.hx;
+    c.bb = (ND == 3) ? c.oy - 1 + c.hy : 0;
+    int const ngm = G.ngh[MD], nm = G.n[MD];
+    c.zs = ngm + seg * L.seglen;
+    c.ze = min(c.zs + L.seglen, ngm + nm);
+    c.col_in_array = c.in_block;
+    c.hancock_ok = c.in_block;
+    c.out_col = c.in_block;
+    if constexpr (ND >= 2) {
+        c.col_in_array = c.col_in_array && c.a < G.nwg[0];
+        c.hancock_ok = c.hancock_ok && c.a <= G.ngh[0] + G.n[0];
+        c.out_col = c.out_col && c.hx >= 1 && c.hx <= TX && c.a < G.ngh[0] + G.n[0];
+    }
+    if constexpr (ND == 3) {
+        c.col_in_array = c.col_in_array && c.bb < G.nwg[1];
+        c.hancock_ok = c.hancock_ok && c.bb <= G.ngh[1] + G.n[1];
+        c.out_col = c.out_col && c.hy >= 1 && c.hy <= TY && c.bb < G.ngh[1] + G.n[1];
+    }
+    c.seg_ok = c.zs < ngm + nm;
+    if constexpr (ND == 1) {
+        c.col_in_array = c.seg_ok;
+        c.hancock_ok = c.seg_ok;
+        c.out_col = c.seg_ok;
+    }
+    return c;
+}
+
 // Minimum waves per SIMD the register allocator must allow.  Measured on MI355X: the 3D
 // FP32-storage kernel gains 23 % at 3 waves (168 VGPRs); the 3D Cartesian FP64 kernel gains 30 %
 // at 2 waves (256 VGPRs, 44 B of spills per lane, with the LDS parking below; two workgroups
@@ -396,52 +460,10 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
     T const dth = T(dth_d);
 
     int const tid = threadIdx.x;
-    // XCD-aware block order: blocks that share an XCD (b % 8) get consecutive tiles.
-    int const nb = gridDim.x;
-    int const b = blockIdx.x;
-    int const q = nb / 8, rem = nb % 8, x8 = b % 8;
-    int const lb = x8 * q + (x8 < rem ? x8 : rem) + b / 8;
-
-    int hx = 0, hy = 0, seg;
-    int ox = 0, oy = 0;
-    if constexpr (ND == 1) {
-        seg = lb * NT + tid;
-    } else {
-        hx = tid % HX;
-        hy = tid / HX;
-        int const tx = lb % L.ntile[0];
-        int const rest = lb / L.ntile[0];
-        int const ty = rest % L.ntile[1];
-        seg = rest / L.ntile[1];
-        ox = G.ngh[0] + tx * TX;
-        oy = (ND == 3) ? G.ngh[1] + ty * TY : 0;
-    }
-    bool const in_block = (ND == 1) || tid < HX * HY;
-    int const a = ox - 1 + hx;                  // column coords (ghost-inclusive)
-    int const bb = (ND == 3) ? oy - 1 + hy : 0;
-    int const ngm = G.ngh[MD], nm = G.n[MD];
-    int const zs = ngm + seg * L.seglen;
-    int const ze = min(zs + L.seglen, ngm + nm);
-
-    bool col_in_array = in_block;               // column exists in memory
-    bool hancock_ok = in_block;                 // column needs a predictor
-    bool out_col = in_block;                    // column holds output cells
-    if constexpr (ND >= 2) {
-        col_in_array = col_in_array && a < G.nwg[0];
-        hancock_ok = hancock_ok && a <= G.ngh[0] + G.n[0];
-        out_col = out_col && hx >= 1 && hx <= TX && a < G.ngh[0] + G.n[0];
-    }
-    if constexpr (ND == 3) {
-        col_in_array = col_in_array && bb < G.nwg[1];
-        hancock_ok = hancock_ok && bb <= G.ngh[1] + G.n[1];
-        out_col = out_col && hy >= 1 && hy <= TY && bb < G.ngh[1] + G.n[1];
-    }
-    bool const seg_ok = zs < ngm + nm;
-    if constexpr (ND == 1) {
-        col_in_array = seg_ok;
-        hancock_ok = seg_ok;
-        out_col = seg_ok;
-    }
+    MarchColumn const mc = march_column<ND, HX, HY, NT>(G, L, blockIdx.x, gridDim.x, tid);
+    int const hx = mc.hx, hy = mc.hy, ox = mc.ox, oy = mc.oy, a = mc.a, bb = mc.bb, zs = mc.zs, ze = mc.ze;
+    bool const in_block = mc.in_block, col_in_array = mc.col_in_array, hancock_ok = mc.hancock_ok, out_col = mc.out_col;
+    bool const seg_ok = mc.seg_ok;
 
     // ring cells of the primitive plane (in-plane distance 2 from the output tile)
     constexpr int NRING = (ND == 3) ? 2 * (PX - 2) + 2 * (PY - 2) : ((ND == 2) ? 2 : 0);
@@ -500,6 +522,11 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
     T old_rho_prev = 0, old_mom_prev = 0, src_prev = 0;
     // pressure fix: the cell of plane h-1 is rewritten by k_pressure_fix_fused, which also supplies its rate
     bool pf_prev = false, pf_cur = false;
+    // set when the fix rewrites some consumed cell of this workgroup (in LDS, not in a register
+    // live across the march)
+    __shared__ int s_pf_any;
+    if (tid == 0) s_pf_any = 0;
+    __syncthreads();
     // passive scalars: mass fractions of planes h-1, h, h+1 and the prefetch; the ring cell's
     // prefetch; the right march face of plane h-1, its partial-density update and the
     // upwinded scalar flux of its left march face (godunov_scheme.hpp:263-290)
@@ -668,7 +695,10 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
                 // pressure recovery formed it
                 r_emin = reduce_combine<kMin>(r_emin, kCarry ? e0 : state_eint<ND>(eos, c0));
                 if constexpr (std::is_same_v<Eos, PerfectGas>) {
-                    if (A.pf) pf_cur = pressure_fix_applies(c0, A.eps_pf);  // pressure_fix.hpp:61-68
+                    if (A.pf) {
+                        pf_cur = pressure_fix_applies(c0, A.eps_pf);  // pressure_fix.hpp:61-68
+                        if (pf_cur) s_pf_any = 1;
+                    }
                 }
             }
             if constexpr (ND >= 2) {
@@ -875,12 +905,14 @@ __global__ __launch_bounds__(NT, (march_min_waves<Eos, ND, GEOM>())) void k_step
     r_emin = block_reduce<kMin>(r_emin, s_red, tid, NT);
 #pragma unroll
     for (int v = 0; v < NV; ++v) r_sum[v] = block_reduce<kSum>(r_sum[v], s_red, tid, NT);
+    static_assert(2 + NV < kPartialStride, "the pressure-fix flag has a slot of its own");
     if (tid == 0 && !A.scalars_only) {
         double* out = A.partials + (int64_t)blockIdx.x * kPartialStride;
         out[0] = r_inv;
         out[1] = r_emin;
 #pragma unroll
         for (int v = 0; v < NV; ++v) out[2 + v] = r_sum[v];
+        out[kPartialStride - 1] = s_pf_any ? 1.0 : 0.0;  // the reductions' barriers ordered every write
     }
 }
 

@@ -9,7 +9,8 @@
 // predictor (same function, same inputs: the same bits), then applies the reference's formula
 // to E_new.  The march kernel leaves these cells out of its CFL maximum; this kernel supplies
 // their rate from the fixed state and the change of their energy times dV, per workgroup, for
-// the device clock's fold.  Cells the fix does not touch cost one read of their state.
+// the device clock's fold.  Cells the fix does not touch cost one read of their state, and a
+// workgroup with no such cell writes identity partials without reducing.
 #include <hip/hip_runtime.h>
 
 #include "march.hpp"
@@ -43,22 +44,35 @@ __device__ __forceinline__ void predictor_at(const GridDev& G, const PerfectGas&
     predict_faces<PerfectGas, ND, GEOM, LIM, 0>(G, eos, dth, dtodv_h, i, j, k, p, nb, fs, x, xnb, xf);
 }
 
-template <int ND, int GEOM, int LIM>
-__global__ __launch_bounds__(256) void k_pressure_fix_fused(GridDev G, PerfectGas eos, StepArrays A, Range3 r, double* __restrict__ fix_partials)
+template <int ND, int GEOM, int LIM, int HX, int HY, int NT>
+__global__ __launch_bounds__(NT) void k_pressure_fix_fused(GridDev G, PerfectGas eos, StepArrays A, FusedLayout L,
+                                                           double* __restrict__ fix_partials)
 {
-    __shared__ double s_red[4];
+    __shared__ double s_red[NT / 64];
     DevClock const* clk = A.clock;
     if (clk->pub.done || clk->pub.status) return;
+    int const b = blockIdx.x, tid = threadIdx.x;
+    double rate = reduce_identity<kMax>();
+    double dE = 0;
+    // the march workgroup b flagged whether any of its cells is rewritten
+    if (A.partials[(int64_t)b * kPartialStride + kPartialStride - 1] == 0.0) {
+        if (tid == 0) {
+            fix_partials[2 * b] = rate;
+            fix_partials[2 * b + 1] = dE;
+        }
+        return;
+    }
     double const dt = clk->pub.dt;
     double const dth = dt / 2;
     double const eps = A.eps_pf;
-    int i, j, k;
-    double rate = reduce_identity<kMax>();
-    double dE = 0;
-    if (range_cell(r, i, j, k)) {
-        int64_t const id = flat(G, i, j, k);
-        State<ND> const cons = load_cons<ND>(A, G, id);
-        if (pressure_fix_applies(cons, eps)) {
+    MarchColumn const mc = march_column<ND, HX, HY, NT>(G, L, b, gridDim.x, tid);
+    if (mc.out_col) {
+        for (int z = mc.zs; z < mc.ze; ++z) {
+            int i, j, k;
+            ijk_of<ND>(mc.a, mc.bb, z, i, j, k);
+            int64_t const id = flat(G, i, j, k);
+            State<ND> const cons = load_cons<ND>(A, G, id);
+            if (!pressure_fix_applies(cons, eps)) continue;
             State<ND> un;
             un[0] = A.rho_new[id];
 #pragma unroll
@@ -84,17 +98,15 @@ __global__ __launch_bounds__(256) void k_pressure_fix_fused(GridDev G, PerfectGa
             };
             double const E_fixed = pressure_fix_energy<ND, GEOM>(G, eos, dt, eps, i, j, k, cons, un, cell, face);
             double const vol = cell_volume<ND, GEOM>(G, i, j, k);
-            dE = (E_fixed - un[ND + 1]) * vol;
+            dE += (E_fixed - un[ND + 1]) * vol;
             A.E_new[id] = E_fixed;
             un[ND + 1] = E_fixed;
-            rate = state_rate<ND, GEOM>(G, eos, i, j, k, un);
+            rate = reduce_combine<kMax>(rate, state_rate<ND, GEOM>(G, eos, i, j, k, un));
         }
     }
-    int const tid = threadIdx.x + kBX * threadIdx.y;
-    rate = block_reduce<kMax>(rate, s_red, tid, kBX * kBY);
-    dE = block_reduce<kSum>(dE, s_red, tid, kBX * kBY);
+    rate = block_reduce<kMax>(rate, s_red, tid, NT);
+    dE = block_reduce<kSum>(dE, s_red, tid, NT);
     if (tid == 0) {
-        int64_t const b = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
         fix_partials[2 * b] = rate;
         fix_partials[2 * b + 1] = dE;
     }
@@ -104,26 +116,29 @@ struct FixLaunch {
     const GridDev& G;
     const PerfectGas& eos;
     const StepArrays& A;
-    const Range3& r;
+    const MarchPlan& P;
     double* partials;
     hipStream_t s;
     template <int ND, int GEOM, int RS, int LIM>
     void run() const
     {
+        using S = MarchShape<ND>;
         if constexpr (RS == kHLL) {  // the fix does not depend on the Riemann solver: one instance per limiter
-            hipLaunchKernelGGL((k_pressure_fix_fused<ND, GEOM, LIM>), range_grid(r), dim3(kBX, kBY), 0, s, G, eos, A, r, partials);
+            if (P.nb == 0) return;
+            hipLaunchKernelGGL((k_pressure_fix_fused<ND, GEOM, LIM, S::HX, S::HY, S::NT>), dim3((unsigned)P.nb), dim3(S::NT), 0, s, G, eos,
+                               A, P.L, partials);
         }
     }
 };
 
 }  // namespace
 
-// The fused step's pressure fix over the interior (range r), after k_step_march; fix_partials
-// receives two doubles per workgroup of range_grid(r).
-void launch_pressure_fix_fused(int nd, int geom, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, const Range3& r,
+// The fused step's pressure fix over the march kernel's workgroups (plan P), after k_step_march;
+// fix_partials receives two doubles per workgroup.
+void launch_pressure_fix_fused(int nd, int geom, int lim, const GridDev& G, const PerfectGas& eos, const StepArrays& A, const MarchPlan& P,
                                double* fix_partials, hipStream_t s)
 {
-    march_visit(nd, geom, kHLL, lim, FixLaunch{G, eos, A, r, fix_partials, s});
+    march_visit(nd, geom, kHLL, lim, FixLaunch{G, eos, A, P, fix_partials, s});
 }
 
 }  // namespace hclpp
