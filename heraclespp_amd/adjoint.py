@@ -38,7 +38,8 @@ def _ptr_array(ts):
 
 @dataclass
 class RolloutSpec:
-    """Grid and scheme of a rollout (Cartesian geometry, HLL or HLLC, any limiter and boundaries)."""
+    """Grid and scheme of a rollout (Cartesian or spherical geometry, HLL or HLLC, any limiter and
+    boundaries)."""
 
     grid: Grid
     reconstruction: str

@@ -987,7 +987,6 @@ int step_adjoint(hclpp_ctx* c, double dt, const double* rho, const double* rhou,
 {
     if (!c) return fail(HCLPP_ERR_ARG, "null context");
     if (c->cfg.precision != HCLPP_FP64) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step takes FP64 storage");
-    if (c->cfg.geom != HCLPP_GEOM_CARTESIAN) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step supports Cartesian geometry only");
     if (c->cfg.riemann != HCLPP_RS_HLL && c->cfg.riemann != HCLPP_RS_HLLC)
         return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step supports the HLL and HLLC solvers only");
     if (c->cfg.nfx > 0) return fail(HCLPP_ERR_UNSUPPORTED, "the reverse step has no passive scalars");
@@ -1007,7 +1006,7 @@ int step_adjoint(hclpp_ctx* c, double dt, const double* rho, const double* rhou,
         HCLPP_HIP(hipMalloc(&c->adj_partials, sizeof(double) * adjoint_partials(c->G, nd)));
     }
     adj::AdjArrays A{rho, rhou, E, c->adj_prim, c->adj_face, lrho_new, lrhou_new, lE_new, lrho, lrhou, lE, dt};
-    *pending = launch_step_adjoint(nd, c->cfg.riemann, c->cfg.limiter, c->G, c->eos, c->bc, A, prims_ready, next_rho, next_rhou, next_E,
+    *pending = launch_step_adjoint(nd, c->cfg.geom, c->cfg.riemann, c->cfg.limiter, c->G, c->eos, c->bc, A, prims_ready, next_rho, next_rhou, next_E,
                                    *pending, fold_now, c->adj_partials, dgamma, c->stream);
     HCLPP_LAUNCHED("reverse step");
     return HCLPP_OK;

@@ -1,5 +1,5 @@
-// adjoint.hip -- reverse-mode hydro step on gfx950 (Cartesian, HLL/HLLC, every limiter,
-// uniform or point-mass gravity, every boundary kind), dt held fixed.
+// adjoint.hip -- reverse-mode hydro step on gfx950 (Cartesian or spherical geometry, HLL/HLLC,
+// every limiter, uniform or point-mass gravity, every boundary kind), dt held fixed.
 //
 // Five passes over 64 x 4 workgroups (x fastest, coalesced SoA planes); each output element
 // is written by exactly one thread, and the dL/dgamma contributions are reduced per
@@ -61,12 +61,12 @@ __global__ __launch_bounds__(256) void k_adj_prim(GridDev G, Range3 r, PerfectGa
 
 // Face states; the first workgroup also folds the `pending` partials the previous reverse step
 // left (the kernel boundary has published them, and this step writes its own only later).
-template <int ND, int LIM>
+template <int ND, int LIM, int GEOM>
 __global__ __launch_bounds__(256) void k_adj_faces(GridDev G, Range3 r, PerfectGas eos, AdjArrays A, const double* __restrict__ partials,
                                                    int64_t pending, double* __restrict__ dgamma)
 {
     int i, j, k;
-    if (range_cell(r, i, j, k)) adj::cell_faces<ND, LIM>(G, eos, A, i, j, k);
+    if (range_cell(r, i, j, k)) adj::cell_faces<ND, LIM, GEOM>(G, eos, A, i, j, k);
     if (pending > 0 && block_lin() == 0) fold_partials(partials, pending, dgamma);
 }
 
@@ -83,28 +83,28 @@ __device__ __forceinline__ bool on_face_range(const GridDev& G, int d, int nd, c
 // Every face direction in one pass: cell c handles the face on its low side along each dim
 // whose face range holds c.  Each face reads and rewrites its own two slots, so no two faces
 // touch the same memory.
-template <int ND, int RS>
+template <int ND, int RS, int GEOM>
 __global__ __launch_bounds__(256) void k_adj_flux(GridDev G, Range3 r, PerfectGas eos, AdjArrays A, double* __restrict__ partials)
 {
     int i, j, k;
     double lg = 0;
     if (range_cell(r, i, j, k)) {
         int const c[3] = {i, j, k};
-        if (on_face_range(G, 0, ND, c)) lg += adj::face_flux<ND, RS, 0>(G, eos, A, i, j, k);
+        if (on_face_range(G, 0, ND, c)) lg += adj::face_flux<ND, RS, 0, GEOM>(G, eos, A, i, j, k);
         if constexpr (ND >= 2)
-            if (on_face_range(G, 1, ND, c)) lg += adj::face_flux<ND, RS, 1>(G, eos, A, i, j, k);
+            if (on_face_range(G, 1, ND, c)) lg += adj::face_flux<ND, RS, 1, GEOM>(G, eos, A, i, j, k);
         if constexpr (ND == 3)
-            if (on_face_range(G, 2, ND, c)) lg += adj::face_flux<ND, RS, 2>(G, eos, A, i, j, k);
+            if (on_face_range(G, 2, ND, c)) lg += adj::face_flux<ND, RS, 2, GEOM>(G, eos, A, i, j, k);
     }
     put_partial(lg, partials);
 }
 
-template <int ND, int LIM>
+template <int ND, int LIM, int GEOM>
 __global__ __launch_bounds__(256) void k_adj_hancock(GridDev G, Range3 r, PerfectGas eos, AdjArrays A, double* __restrict__ partials)
 {
     int i, j, k;
     double lg = 0;
-    if (range_cell(r, i, j, k)) lg = adj::cell_hancock<ND, LIM>(G, eos, A, i, j, k);
+    if (range_cell(r, i, j, k)) lg = adj::cell_hancock<ND, LIM, GEOM>(G, eos, A, i, j, k);
     put_partial(lg, partials);
 }
 
@@ -178,15 +178,15 @@ int64_t blocks_of(const Range3& r)
     return (int64_t)g.x * g.y * g.z;
 }
 
-template <int ND>
+template <int ND, int GEOM>
 void launch_flux(int rs, const GridDev& G, const AdjRanges& R, const PerfectGas& eos, const AdjArrays& A, double* partials, int64_t& off,
                  hipStream_t s)
 {
     Range3 const& r = R.flux;
     if (rs == kHLL)
-        hipLaunchKernelGGL((k_adj_flux<ND, kHLL>), range_grid(r), dim3(kBX, kBY), 0, s, G, r, eos, A, partials + off);
+        hipLaunchKernelGGL((k_adj_flux<ND, kHLL, GEOM>), range_grid(r), dim3(kBX, kBY), 0, s, G, r, eos, A, partials + off);
     else
-        hipLaunchKernelGGL((k_adj_flux<ND, kHLLC>), range_grid(r), dim3(kBX, kBY), 0, s, G, r, eos, A, partials + off);
+        hipLaunchKernelGGL((k_adj_flux<ND, kHLLC, GEOM>), range_grid(r), dim3(kBX, kBY), 0, s, G, r, eos, A, partials + off);
     off += blocks_of(r);
 }
 
@@ -205,6 +205,9 @@ int64_t adjoint_partials(const GridDev& G, int nd)
     case kMinmod: { constexpr int LIM = kMinmod; BODY; } break;       \
     default: { constexpr int LIM = kVanAlbada; BODY; } break;         \
     }
+#define HCLPP_ADJ_GEOM_SWITCH(geom, BODY)                                       \
+    if (geom == kSpherical) { constexpr int GEOM = kSpherical; BODY; }          \
+    else { constexpr int GEOM = kCartesian; BODY; }
 #define HCLPP_ADJ_ND_SWITCH(nd, BODY)                  \
     switch (nd) {                                      \
     case 1: { constexpr int ND = 1; BODY; } break;     \
@@ -212,7 +215,7 @@ int64_t adjoint_partials(const GridDev& G, int nd)
     default: { constexpr int ND = 3; BODY; } break;    \
     }
 
-int64_t launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const AdjArrays& A,
+int64_t launch_step_adjoint(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const AdjArrays& A,
                             bool prims_ready, const double* next_rho, const double* next_rhou, const double* next_E, int64_t pending,
                             bool fold_now, double* partials, double* dgamma, hipStream_t s)
 {
@@ -224,18 +227,18 @@ int64_t launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const Per
     // dgamma does not depend on how a run of reverse steps is split into calls.
     Range3 const& rg = R.ring2;
     int64_t off = 0;
-    HCLPP_ADJ_ND_SWITCH(nd, {
+    HCLPP_ADJ_ND_SWITCH(nd, HCLPP_ADJ_GEOM_SWITCH(geom, {
         if (!prims_ready) hipLaunchKernelGGL(k_adj_prim<ND>, range_grid(R.ring2), blk, 0, s, G, R.ring2, eos, A);
-        HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_faces<ND, LIM>), range_grid(R.ring1), blk, 0, s, G, R.ring1, eos, A,
+        HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_faces<ND, LIM, GEOM>), range_grid(R.ring1), blk, 0, s, G, R.ring1, eos, A,
                                                      partials, pending, dgamma));
-        (launch_flux<ND>)(rs, G, R, eos, A, partials, off, s);
-        HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_hancock<ND, LIM>), range_grid(R.ring1), blk, 0, s, G, R.ring1, eos, A,
+        (launch_flux<ND, GEOM>)(rs, G, R, eos, A, partials, off, s);
+        HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_hancock<ND, LIM, GEOM>), range_grid(R.ring1), blk, 0, s, G, R.ring1, eos, A,
                                                      partials + off));
         off += blocks_of(R.ring1);
         HCLPP_ADJ_LIM_SWITCH(lim, hipLaunchKernelGGL((k_adj_gather<ND, LIM>), range_grid(rg), blk, 0, s, G, rg, R.interior, eos, bc, A, nx,
                                                      partials + off));
         off += blocks_of(rg);
-    });
+    }));
     if (!fold_now) return off;
     launch_adjoint_fold(partials, off, dgamma, s);
     return 0;

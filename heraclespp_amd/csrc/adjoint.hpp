@@ -109,14 +109,23 @@ __host__ __device__ __forceinline__ void flux_vjp(const S<ND>& p, int n, double 
     lg += -aE * un * P * igm1 * igm1;
 }
 
-// c = sqrt(gamma P / rho)
+// c = sqrt(gamma P / rho).  A zero adjoint contributes nothing, also where c is not a number: the
+// forward's wave-speed min/max (fmin/fmax) pass over a NaN sound speed (a state with P < 0, as the
+// Hancock state of a ghost cell whose volume is negative can be), and the branch it takes sends
+// no adjoint to that speed.
 __host__ __device__ __forceinline__ void sound_vjp(double rho, double P, double gamma, double c, double lc_, double& lrho, double& lP, double& lg)
 {
+    if (lc_ == 0) return;
     double const i2rc = 1 / (2 * rho * c);  // c / (2 rho) = c^2 / (2 rho c)
     lrho += -lc_ * c * c * i2rc;
     lP += lc_ * gamma * i2rc;
     lg += lc_ * P * i2rc;
 }
+
+// Which operand fmin(a, b) / fmax(a, b) returns: the first unless it is the larger / smaller one
+// or a NaN (fmin and fmax return the other operand when one is a NaN).
+__host__ __device__ __forceinline__ bool fmin_takes_first(double a, double b) { return std::isnan(b) || (!std::isnan(a) && a <= b); }
+__host__ __device__ __forceinline__ bool fmax_takes_first(double a, double b) { return std::isnan(b) || (!std::isnan(a) && a >= b); }
 
 // slope limiter: derivative of the limited slope w.r.t. (diffR, diffL) on the forward branch
 template <int LIM>
@@ -184,8 +193,8 @@ __host__ __device__ __forceinline__ void hll_vjp(const S<ND>& cL, const S<ND>& c
     double const lwsR = wsR > 0 ? lpR : 0.;
     S<ND> lpL = zero<ND>(), lpRs = zero<ND>();
     double lsL = 0, lsR = 0;
-    if (aL <= bL) { lpL[1 + n] += lwsL; lsL -= lwsL; } else { lpRs[1 + n] += lwsL; lsR -= lwsL; }
-    if (aR >= bR) { lpL[1 + n] += lwsR; lsL += lwsR; } else { lpRs[1 + n] += lwsR; lsR += lwsR; }
+    if (fmin_takes_first(aL, bL)) { lpL[1 + n] += lwsL; lsL -= lwsL; } else { lpRs[1 + n] += lwsL; lsR -= lwsL; }
+    if (fmax_takes_first(aR, bR)) { lpL[1 + n] += lwsR; lsL += lwsR; } else { lpRs[1 + n] += lwsR; lsR += lwsR; }
     flux_vjp(pL, n, g, lfL, lpL, lg);
     flux_vjp(pR, n, g, lfR, lpRs, lg);
     sound_vjp(pL[0], pL[ND + 1], g, sL, lsL, lpL[0], lpL[ND + 1], lg);
@@ -306,8 +315,8 @@ __host__ __device__ __forceinline__ void hllc_vjp(const S<ND>& cL, const S<ND>& 
     }
     // wave speeds
     double lsL = 0, lsR = 0;
-    if (aL <= bL) { lpL[1 + n] += lwsL; lsL -= lwsL; } else { lpR[1 + n] += lwsL; lsR -= lwsL; }
-    if (aR >= bR) { lpL[1 + n] += lwsR; lsL += lwsR; } else { lpR[1 + n] += lwsR; lsR += lwsR; }
+    if (fmin_takes_first(aL, bL)) { lpL[1 + n] += lwsL; lsL -= lwsL; } else { lpR[1 + n] += lwsL; lsR -= lwsL; }
+    if (fmax_takes_first(aR, bR)) { lpL[1 + n] += lwsR; lsL += lwsR; } else { lpR[1 + n] += lwsR; lsR += lwsR; }
     sound_vjp(pL[0], PL, g, sL, lsL, lpL[0], lpL[ND + 1], lg);
     sound_vjp(pR[0], PR, g, sR, lsR, lpR[0], lpR[ND + 1], lg);
     prim_vjp(cL, g, lpL, lcL, lg);

@@ -1,11 +1,15 @@
 // adjoint_step.hpp -- per-cell and per-face bodies of the reverse-mode hydro step.
 //
-// The forward step (Cartesian) is
+// The forward step is
 //   Ug = ghost_fill(U);  p = prim(Ug)                         (all cells)
 //   pf[s][d] = p -/+ dl/2 * slope_d(p)                        (cells within 1 of the interior)
-//   Uf[s][d] = cons(pf[s][d]) + dth/dV sum_e (F_e(pf[0][e]) dS - F_e(pf[1][e]) dS+) + gravity(dth)
-//   U'(c) = U(c) + dt/dV sum_d (F*(face c-1/2) dS - F*(face c+1/2) dS+) + gravity(dt)
-// with dth = dt / 2 and F* the Riemann flux of (Uf[1][d] of the left cell, Uf[0][d] of the right one).
+//   Uf[s][d] = cons(pf[s][d]) + sum_e [dth/dV (F_e(pf[0][e]) dS - F_e(pf[1][e]) dS+) + Q_e(pf[0][e], pf[1][e])]
+//              + gravity(dth)
+//   U'(c) = U(c) + sum_d [dt/dV (F*(face c-1/2) dS - F*(face c+1/2) dS+) + Q_d(prim(Uf[0][d]), prim(Uf[1][d]))]
+//           + gravity(dt)
+// with dth = dt / 2, F* the Riemann flux of (Uf[1][d] of the left cell, Uf[0][d] of the right one)
+// and Q_d the spherical geometric sources of dims 0 and 1 (zero in Cartesian geometry), formed
+// from the face states of the cell's own faces (hydro/source_terms.hpp:18-77).
 // Given the adjoint lambda' of U' on the interior, the reverse step runs five passes; each
 // writes every output element exactly once (no atomics, results are deterministic):
 //   prim   (cells within 2 of the interior)   p = prim(Ug)
@@ -13,9 +17,12 @@
 //   flux   (faces touching the interior, every dim in one pass)
 //                               lambda_F = dt/dV (lambda'(right) dS - lambda'(left) dS+), Riemann VJP,
 //                               the adjoints of the two face states overwrite their slots
-//   hancock (ring-1 cells)      adjoint of the Hancock predictor (face-state adjoints -> face-prim
-//                               adjoints), then of the limited slopes: the cell's own prim adjoint
-//                               and the shares owed to its +-1 neighbours, overwriting the slots
+//   hancock (ring-1 cells)      in spherical geometry, first the Godunov sources' adjoint (interior
+//                               cells: the face states Uf recomputed, their adjoint from lambda'(c));
+//                               then the adjoint of the Hancock predictor (face-state adjoints ->
+//                               face-prim adjoints, the predictor's sources included), then of the
+//                               limited slopes: the cell's own prim adjoint and the shares owed to
+//                               its +-1 neighbours, overwriting the slots
 //   gather (interior cells)     cell-prim adjoint = own part + the neighbours' shares, prim -> cons
 //                               VJP, then the transposed ghost fill (every ghost cell whose value
 //                               was copied from this cell adds its adjoint back, sign-flipped where
@@ -121,26 +128,65 @@ __host__ __device__ __forceinline__ void face_prims(const GridDev& G, const doub
     }
 }
 
-// ---------------------------------------------------------------- pass 1: primitives of every cell
+// Adjoint of the spherical source terms of dim `dim` (hydro/source_terms.hpp:18-77, added as in
+// extrapolation_reconstruction.hpp:145-303 and godunov_scheme.hpp:165-256): lc is the adjoint of
+// the conservative components the terms are added to (subtracted for the Coriolis terms), pL and
+// pR the primitive face states they are formed from; accumulates into lpL and lpR.
 template <int ND>
-__host__ __device__ __forceinline__ void cell_prim(const GridDev& G, const PerfectGas& eos, const AdjArrays& A, int i, int j, int k)
+__host__ __device__ __forceinline__ void spherical_sources_vjp(const GridDev& G, int dim, int i, int j, double dtodv, double dS, double dSp,
+                                                               const S<ND>& pL, const S<ND>& pR, const S<ND>& lc, S<ND>& lpL, S<ND>& lpR)
 {
-    int64_t const id = flat(G, i, j, k);
-    store_planes<ND>(A.prim, id, G.plane, 0, cons_to_prim(load_cons<ND>(A.rho, A.rhou, A.E, id, G.plane), eos));
+    double const w = dtodv * 0.5 * (dSp - dS);  // the pressure term is w (PL + PR)
+    // centrifugal term of velocity component o, added to component m: w/2 (rL uL^2 + rR uR^2)
+    auto centrifugal = [&](int m, int o, double scale) {
+        double const k = 0.5 * w * scale * lc[m];
+        lpL[0] += k * pL[o] * pL[o];
+        lpR[0] += k * pR[o] * pR[o];
+        lpL[o] += k * 2 * pL[0] * pL[o];
+        lpR[o] += k * 2 * pR[0] * pR[o];
+    };
+    // Coriolis term subtracted from component m: q (rR uRo uRr dS+ + rL uLo uLr dS), with r the
+    // component along `r` and o the one along `o`
+    auto coriolis = [&](int m, int r, int o, double q) {
+        double const k = -q * lc[m];
+        lpR[0] += k * pR[o] * pR[r] * dSp;
+        lpR[o] += k * pR[0] * pR[r] * dSp;
+        lpR[r] += k * pR[0] * pR[o] * dSp;
+        lpL[0] += k * pL[o] * pL[r] * dS;
+        lpL[o] += k * pL[0] * pL[r] * dS;
+        lpL[r] += k * pL[0] * pL[o] * dS;
+    };
+    if (dim == 0) {
+        lpL[ND + 1] += w * lc[1];  // pressure: dtodv (PL + PR) / 2 (dS+ - dS) = w (PL + PR) on the radial momentum
+        lpR[ND + 1] += w * lc[1];
+        if constexpr (ND >= 2) {
+            double const x0 = G.x[0][i], x0p = G.x[0][i + 1];
+            double const q = fdiv(dtodv * (x0p - x0), x0p + x0);
+            centrifugal(1, 2, 1.0);
+            coriolis(2, 1, 2, q);
+            if constexpr (ND == 3) {
+                centrifugal(1, 3, 1.0);
+                coriolis(3, 1, 3, q);
+            }
+        }
+    } else if (dim == 1) {
+        lpL[ND + 1] += w * lc[2];
+        lpR[ND + 1] += w * lc[2];
+        if constexpr (ND == 3) {
+            double const sn = G.sin_node[j], snp = G.sin_node[j + 1];
+            centrifugal(2, 3, G.cot_mid[j]);
+            coriolis(3, 2, 3, fdiv(dtodv * (snp - sn), snp + sn));
+        }
+    }
 }
 
-// ---------------------------------------------------------------- pass 2: Hancock face states (ring-1 cells)
-// extrapolation_reconstruction.hpp:90-325 with dt/2, in the forward's accumulation order.
-template <int ND, int LIM>
-__host__ __device__ __forceinline__ void cell_faces(const GridDev& G, const PerfectGas& eos, const AdjArrays& A, int i, int j, int k)
+// Hancock face states of cell (i, j, k) from its face primitives pf, dth = dt / 2:
+// extrapolation_reconstruction.hpp:90-312 in the forward's accumulation order (per dim: the flux
+// difference on every face, the spherical sources of that dim, gravity).
+template <int ND, int GEOM>
+__host__ __device__ __forceinline__ void hancock_states(const GridDev& G, const PerfectGas& eos, double dth, int i, int j, int k,
+                                                        const S<ND> (&pf)[2][ND], S<ND> (&fs)[2][ND])
 {
-    int const c[3] = {i, j, k};
-    int64_t const id = flat(G, i, j, k);
-    int64_t const pl = G.plane;
-    S<ND> const p = load_planes<ND>(A.prim, id, pl, 0);
-    S<ND> pf[2][ND];
-    face_prims<ND, LIM>(G, A.prim, c, p, pf);
-    S<ND> fs[2][ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d)
 #pragma unroll
@@ -154,20 +200,48 @@ __host__ __device__ __forceinline__ void cell_faces(const GridDev& G, const Perf
 #pragma unroll
             for (int v = 0; v < ND; ++v) rhou_old[s][d][v] = fs[s][d][1 + v];
         }
-    double const dth = A.dt / 2;
+    double const dtodv = fdiv(dth, cell_volume<ND, GEOM>(G, i, j, k));
+    double const x0 = G.x[0][i], x0p = G.x[0][i + 1];
 #pragma unroll
     for (int dim = 0; dim < ND; ++dim) {
-        S<ND> const fL = flux_of_prim(pf[0][dim], dim, eos);
-        S<ND> const fR = flux_of_prim(pf[1][dim], dim, eos);
-        double const dtodv = fdiv(dth, cell_volume<ND, kCartesian>(G, i, j, k));
-        double const dS = face_area<ND, kCartesian>(G, dim, i, j, k);
-        double const dSp = face_area<ND, kCartesian>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2));
+        S<ND> const& pL = pf[0][dim];
+        S<ND> const& pR = pf[1][dim];
+        S<ND> const fL = flux_of_prim(pL, dim, eos);
+        S<ND> const fR = flux_of_prim(pR, dim, eos);
+        double const dS = face_area<ND, GEOM>(G, dim, i, j, k);
+        double const dSp = face_area<ND, GEOM>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2));
 #pragma unroll
         for (int pos = 0; pos < ND; ++pos)
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
                 for (int v = 0; v < ND + 2; ++v) fs[s][pos][v] += dtodv * ((fL[v] * dS) - (fR[v] * dSp));
+        if constexpr (GEOM == kSpherical) {
+#pragma unroll
+            for (int pos = 0; pos < ND; ++pos)
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    if (dim == 0) {
+                        fs[s][pos][1] += src_pressure(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+                        if constexpr (ND >= 2) {
+                            fs[s][pos][1] += src_centrifugal(dtodv, pL[0], pR[0], pL[2], pR[2], dS, dSp);
+                            fs[s][pos][2] -= src_coriolis_r(dtodv, x0, x0p, pL[0], pR[0], pL[1], pR[1], pL[2], pR[2], dS, dSp);
+                        }
+                        if constexpr (ND == 3) {
+                            fs[s][pos][1] += src_centrifugal(dtodv, pL[0], pR[0], pL[3], pR[3], dS, dSp);
+                            fs[s][pos][3] -= src_coriolis_r(dtodv, x0, x0p, pL[0], pR[0], pL[1], pR[1], pL[3], pR[3], dS, dSp);
+                        }
+                    }
+                    if (dim == 1) {
+                        fs[s][pos][2] += src_pressure(dtodv, pL[ND + 1], pR[ND + 1], dS, dSp);
+                        if constexpr (ND == 3) {
+                            fs[s][pos][2] += src_centrifugal_th(dtodv, G.cot_mid[j], pL[0], pR[0], pL[3], pR[3], dS, dSp);
+                            fs[s][pos][3] -= src_coriolis_th(dtodv, G.sin_node[j], G.sin_node[j + 1], pL[0], pR[0], pL[2], pR[2], pL[3], pR[3],
+                                                             dS, dSp);
+                        }
+                    }
+                }
+        }
         double const g = gravity(G, i, dim);
 #pragma unroll
         for (int pos = 0; pos < ND; ++pos)
@@ -177,6 +251,29 @@ __host__ __device__ __forceinline__ void cell_faces(const GridDev& G, const Perf
                 fs[s][pos][ND + 1] += dth * g * rhou_old[s][pos][dim];
             }
     }
+}
+
+// ---------------------------------------------------------------- pass 1: primitives of every cell
+template <int ND>
+__host__ __device__ __forceinline__ void cell_prim(const GridDev& G, const PerfectGas& eos, const AdjArrays& A, int i, int j, int k)
+{
+    int64_t const id = flat(G, i, j, k);
+    store_planes<ND>(A.prim, id, G.plane, 0, cons_to_prim(load_cons<ND>(A.rho, A.rhou, A.E, id, G.plane), eos));
+}
+
+// ---------------------------------------------------------------- pass 2: Hancock face states (ring-1 cells)
+// extrapolation_reconstruction.hpp:90-325 with dt/2, in the forward's accumulation order.
+template <int ND, int LIM, int GEOM>
+__host__ __device__ __forceinline__ void cell_faces(const GridDev& G, const PerfectGas& eos, const AdjArrays& A, int i, int j, int k)
+{
+    int const c[3] = {i, j, k};
+    int64_t const id = flat(G, i, j, k);
+    int64_t const pl = G.plane;
+    S<ND> const p = load_planes<ND>(A.prim, id, pl, 0);
+    S<ND> pf[2][ND];
+    face_prims<ND, LIM>(G, A.prim, c, p, pf);
+    S<ND> fs[2][ND];
+    hancock_states<ND, GEOM>(G, eos, A.dt / 2, i, j, k, pf, fs);
 #pragma unroll
     for (int d = 0; d < ND; ++d)
 #pragma unroll
@@ -187,7 +284,7 @@ __host__ __device__ __forceinline__ void cell_faces(const GridDev& G, const Perf
 // Face between cell c - e_d and cell c, with c_d in [ng, ng + n] and c interior along the other dims.
 // The face direction is a template parameter: the Riemann VJP indexes state components by it,
 // and a run-time index would push the register arrays to scratch memory.
-template <int ND, int RS, int d>
+template <int ND, int RS, int d, int GEOM>
 __host__ __device__ __forceinline__ double face_flux(const GridDev& G, const PerfectGas& eos, const AdjArrays& A, int i, int j, int k)
 {
     constexpr int NV = ND + 2;
@@ -200,14 +297,14 @@ __host__ __device__ __forceinline__ double face_flux(const GridDev& G, const Per
     S<ND> lF = zero<ND>();
     // godunov_scheme.hpp:110-292: U'(c) += dt/dV F dS;  U'(c - e_d) -= dt/dV F dS+
     if (inside(G, d, c[d])) {
-        double const w = fdiv(A.dt, cell_volume<ND, kCartesian>(G, i, j, k)) * face_area<ND, kCartesian>(G, d, i, j, k);
+        double const w = fdiv(A.dt, cell_volume<ND, GEOM>(G, i, j, k)) * face_area<ND, GEOM>(G, d, i, j, k);
         S<ND> const l = load_cons<ND>(A.lrho_new, A.lrhou_new, A.lE_new, id, pl);
 #pragma unroll
         for (int v = 0; v < NV; ++v) lF[v] += w * l[v];
     }
     if (inside(G, d, c[d] - 1)) {
         int const im = i - (d == 0), jm = j - (d == 1), km = k - (d == 2);
-        double const w = fdiv(A.dt, cell_volume<ND, kCartesian>(G, im, jm, km)) * face_area<ND, kCartesian>(G, d, i, j, k);
+        double const w = fdiv(A.dt, cell_volume<ND, GEOM>(G, im, jm, km)) * face_area<ND, GEOM>(G, d, i, j, k);
         S<ND> const l = load_cons<ND>(A.lrho_new, A.lrhou_new, A.lE_new, idm, pl);
 #pragma unroll
         for (int v = 0; v < NV; ++v) lF[v] -= w * l[v];
@@ -261,7 +358,7 @@ __host__ __device__ __forceinline__ void slope_adjoint(const GridDev& G, const d
 }
 
 // ---------------------------------------------------------------- pass 4: Hancock adjoint (ring-1 cells)
-template <int ND, int LIM>
+template <int ND, int LIM, int GEOM>
 __host__ __device__ __forceinline__ double cell_hancock(const GridDev& G, const PerfectGas& eos, const AdjArrays& A, int i, int j, int k)
 {
     constexpr int NV = ND + 2;
@@ -289,6 +386,31 @@ __host__ __device__ __forceinline__ double cell_hancock(const GridDev& G, const 
         S<ND> pf[2][ND];
         face_prims<ND, LIM>(G, A.prim, c, p, pf);
         double const dth = A.dt / 2;
+        if constexpr (GEOM == kSpherical) {
+            // godunov_scheme.hpp:165-256: the update of an interior cell adds the sources of dims 0
+            // and 1 formed from the primitives of its own face states Uf: their adjoint joins the
+            // face-state adjoints before the predictor is reversed
+            bool interior = true;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) interior = interior && inside(G, d, c[d]);
+            if (interior) {
+                S<ND> fs[2][ND];
+                hancock_states<ND, GEOM>(G, eos, dth, i, j, k, pf, fs);
+                S<ND> const ln = load_cons<ND>(A.lrho_new, A.lrhou_new, A.lE_new, id, pl);
+                double const dtodv = fdiv(A.dt, cell_volume<ND, GEOM>(G, i, j, k));
+#pragma unroll
+                for (int dim = 0; dim < (ND < 2 ? ND : 2); ++dim) {
+                    S<ND> const pL = cons_to_prim(fs[0][dim], eos);
+                    S<ND> const pR = cons_to_prim(fs[1][dim], eos);
+                    double const dS = face_area<ND, GEOM>(G, dim, i, j, k);
+                    double const dSp = face_area<ND, GEOM>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2));
+                    S<ND> lpL = zero<ND>(), lpR = zero<ND>();
+                    spherical_sources_vjp<ND>(G, dim, i, j, dtodv, dS, dSp, pL, pR, ln, lpL, lpR);
+                    prim_vjp(fs[0][dim], eos.gamma, lpL, lam[0][dim], lg);
+                    prim_vjp(fs[1][dim], eos.gamma, lpR, lam[1][dim], lg);
+                }
+            }
+        }
         S<ND> lD = zero<ND>();
 #pragma unroll
         for (int pos = 0; pos < ND; ++pos)
@@ -306,11 +428,14 @@ __host__ __device__ __forceinline__ double cell_hancock(const GridDev& G, const 
                 }
                 cons_vjp(pf[s][pos], eos.gamma, lc, lpf[s][pos], lg);
             }
-        double const dtodv = fdiv(dth, cell_volume<ND, kCartesian>(G, i, j, k));
+        double const dtodv = fdiv(dth, cell_volume<ND, GEOM>(G, i, j, k));
 #pragma unroll
         for (int dim = 0; dim < ND; ++dim) {
-            double const dS = face_area<ND, kCartesian>(G, dim, i, j, k);
-            double const dSp = face_area<ND, kCartesian>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2));
+            double const dS = face_area<ND, GEOM>(G, dim, i, j, k);
+            double const dSp = face_area<ND, GEOM>(G, dim, i + (dim == 0), j + (dim == 1), k + (dim == 2));
+            // the predictor's sources of this dim were added to every face state: their adjoint is lD
+            if constexpr (GEOM == kSpherical) spherical_sources_vjp<ND>(G, dim, i, j, dtodv, dS, dSp, pf[0][dim], pf[1][dim], lD, lpf[0][dim],
+                                                                      lpf[1][dim]);
             S<ND> a, b;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -454,7 +579,7 @@ namespace hclpp {
 // partials before returning (returns 0), else return their number for the next step's `pending`.
 // launch_adjoint_fold adds n partials to dgamma on its own.
 int64_t adjoint_partials(const GridDev& G, int nd);
-int64_t launch_step_adjoint(int nd, int rs, int lim, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const adj::AdjArrays& A,
+int64_t launch_step_adjoint(int nd, int geom, int rs, int lim, const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const adj::AdjArrays& A,
                             bool prims_ready, const double* next_rho, const double* next_rhou, const double* next_E, int64_t pending,
                             bool fold_now, double* partials, double* dgamma, hipStream_t s);
 void launch_adjoint_fold(const double* partials, int64_t n, double* dgamma, hipStream_t s);

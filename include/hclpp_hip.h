@@ -269,8 +269,9 @@ int hclpp_cast_f32_to_f64(hclpp_ctx* ctx, const float* in, double* out, long lon
 
 /* ---- reverse mode (no counterpart in the reference: the derivative of hclpp_step) ----
  * The step is differentiated with dt held fixed; the caller replays the forward dt
- * schedule (recorded with hclpp_clock_record_dt) through hclpp_step_dt.  Cartesian
- * geometry, HLL or HLLC, any limiter, uniform or point-mass gravity, any boundary
+ * schedule (recorded with hclpp_clock_record_dt) through hclpp_step_dt.  Cartesian or
+ * spherical geometry (the geometric sources of the predictor and of the update are
+ * differentiated), HLL or HLLC, any limiter, uniform or point-mass gravity, any boundary
  * kinds, FP64 storage, nfx = 0, 2 <= ng <= 4; otherwise HCLPP_ERR_UNSUPPORTED. */
 
 /* hclpp_step with the step length dt instead of the clock's CFL dt (t and iter advance

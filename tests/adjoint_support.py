@@ -31,7 +31,7 @@ def host_lib():
         L = ctypes.CDLL(HOST_LIB)
         dp = ctypes.POINTER(ctypes.c_double)
         L.adjh_step.restype = ctypes.c_int
-        L.adjh_step.argtypes = ([ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(dp),
+        L.adjh_step.argtypes = ([ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(dp),
                                 ctypes.POINTER(ctypes.c_int), ctypes.c_double, dp, ctypes.c_double] + [dp] * 9 + [dp])
         _host = L
     return _host
@@ -43,20 +43,22 @@ def _p(a):
 
 
 class Problem:
-    """A small Cartesian problem: oracle context, nodes, boundary kinds, gravity."""
+    """A small problem (Cartesian, or spherical on [xmin, xmax]): oracle context, nodes, boundary
+    kinds, gravity."""
 
-    def __init__(self, ndim, n, *, limiter, riemann, bc, g=(0.0, 0.0, 0.0), gamma=1.4, xmax=(1.0, 1.0, 1.0)):
+    def __init__(self, ndim, n, *, limiter, riemann, bc, g=(0.0, 0.0, 0.0), gamma=1.4, xmax=(1.0, 1.0, 1.0),
+                 geom="Cartesian", xmin=(0.0, 0.0, 0.0)):
         self.ndim = ndim
         self.n = [int(v) for v in (list(n) + [1, 1, 1])[:3]]
         self.limiter, self.riemann, self.bc, self.g, self.gamma = limiter, riemann, list(bc), tuple(g), gamma
-        self.xmax = xmax
+        self.geom, self.xmin, self.xmax = geom, xmin, xmax
         self.o = self.oracle(gamma)
         self.ng = 2
-        self.nodes = [orc.regular_nodes(self.n[d], 2 if d < ndim else 0, 0.0, xmax[d]) for d in range(3)]
+        self.nodes = [orc.regular_nodes(self.n[d], 2 if d < ndim else 0, xmin[d], xmax[d]) for d in range(3)]
 
     def oracle(self, gamma):
-        return orc.Oracle(self.ndim, self.n, limiter=self.limiter, riemann=self.riemann, gamma=gamma, bc=self.bc,
-                          g=self.g, xmax=self.xmax)
+        return orc.Oracle(self.ndim, self.n, geom=self.geom, limiter=self.limiter, riemann=self.riemann, gamma=gamma,
+                          bc=self.bc, g=self.g, xmin=self.xmin, xmax=self.xmax)
 
     def interior(self):
         sl = [slice(None)] * 3
@@ -113,7 +115,8 @@ class Problem:
         bcs = (ctypes.c_int * 6)(*[orc.BCS[b] for b in self.bc] + [0] * (6 - len(self.bc)))
         g = np.ascontiguousarray(self.g, dtype=np.float64)
         dg = np.zeros(1)
-        rc = L.adjh_step(self.ndim, orc.SOLVERS[self.riemann], orc.LIMITERS[self.limiter], n, 2, xs, bcs, self.gamma, _p(g),
+        rc = L.adjh_step(self.ndim, orc.GEOMS[self.geom], orc.SOLVERS[self.riemann], orc.LIMITERS[self.limiter], n, 2, xs, bcs,
+                         self.gamma, _p(g),
                          dt, _p(rho), _p(rhou), _p(E), _p(lrho_new), _p(lrhou_new), _p(lE_new), _p(lr), _p(lm), _p(le),
                          _p(dg))
         assert rc == 0

@@ -1,6 +1,8 @@
 """Reverse-mode step, checked on the CPU: the host build of the GPU kernel bodies
 (adjoint_step.hpp) against central finite differences of the oracle's fixed-dt step
-(the reference kernel sequence).  The adjoint holds dt fixed, so the finite differences
+(the reference kernel sequence), in Cartesian and spherical geometry (the spherical cases
+differentiate the geometric sources of the predictor and of the update,
+source_terms.hpp:18-77).  The adjoint holds dt fixed, so the finite differences
 do too.  Non-smooth points (limiter sign tests, wave-speed min/max, HLLC branch choices)
 are differentiated on the branch the state takes; a branch decision flipping inside the
 FD step would show as a mismatch far above the 1e-8 tolerance, and the seeded states
@@ -11,8 +13,11 @@ import pytest
 
 from tests.adjoint_support import Problem
 
+SPH = dict(geom="Spherical", xmin=(0.5, 0.3, 0.0), xmax=(1.5, np.pi - 0.3, 2 * np.pi))
+SPH_BC = ["Reflexive", "NullGradient", "Reflexive", "Reflexive", "Periodic", "Periodic"]
+
 CASES = [
-    # id, ndim, n, limiter, riemann, bc, g[, x-velocity offset]
+    # id, ndim, n, limiter, riemann, bc, g[, x-velocity offset[, geometry]]
     ("1d_minmod_hll_ng_refl_g", 1, [12], "Minmod", "HLL", ["NullGradient", "Reflexive"], (-0.3, 0, 0)),
     ("1d_vanleer_hllc_periodic", 1, [10], "VanLeer", "HLLC", ["Periodic", "Periodic"], (0, 0, 0)),
     ("2d_vanalbada_hllc_mixed_g", 2, [9, 7], "VanAlbada", "HLLC", ["Periodic", "Periodic", "Reflexive", "NullGradient"],
@@ -27,6 +32,11 @@ CASES = [
     # supersonic along x (u ~ 2.5 > c): HLL's clipped wave speeds and HLLC's upwind-flux branch
     ("2d_vanleer_hllc_supersonic", 2, [8, 6], "VanLeer", "HLLC", ["Periodic"] * 4, (0, 0, 0), 2.5),
     ("1d_minmod_hll_supersonic", 1, [10], "Minmod", "HLL", ["Periodic"] * 2, (0, 0, 0), -2.5),
+    # spherical (r, theta, phi): reflexive r_min and poles, free r_max, periodic phi
+    ("1d_sph_vanleer_hllc", 1, [12], "VanLeer", "HLLC", SPH_BC[:2], (0, 0, 0), 0.0, SPH),
+    ("2d_sph_minmod_hll_g", 2, [8, 6], "Minmod", "HLL", SPH_BC[:4], (-0.2, 0, 0), 0.0, SPH),
+    ("3d_sph_vanalbada_hllc", 3, [5, 4, 4], "VanAlbada", "HLLC", SPH_BC, (0, 0, 0), 0.0, SPH),
+    ("3d_sph_vanleer_hll", 3, [4, 5, 4], "VanLeer", "HLL", SPH_BC, (0, 0, 0), 0.0, SPH),
 ]
 
 
@@ -37,7 +47,7 @@ def _dot(p, a, b):
 
 def _setup(case, seed):
     _, nd, n, lim, rs, bc, g = case[:7]
-    p = Problem(nd, n, limiter=lim, riemann=rs, bc=bc, g=g)
+    p = Problem(nd, n, limiter=lim, riemann=rs, bc=bc, g=g, **(case[8] if len(case) > 8 else {}))
     U = p.random_state(seed, u0=case[7] if len(case) > 7 else 0.0)
     dt = p.stable_dt(*U)
     rng = np.random.default_rng(seed + 100)
@@ -86,7 +96,8 @@ def test_host_adjoint_gamma_matches_fd(case):
     assert abs(fd - dg) <= 2e-8 * max(1.0, abs(fd)), (fd, dg)
 
 
-@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[5], CASES[8]], ids=[CASES[i][0] for i in (0, 2, 5, 8)])
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[5], CASES[8], CASES[11], CASES[12]],
+                         ids=[CASES[i][0] for i in (0, 2, 5, 8, 11, 12)])
 def test_host_adjoint_every_cell_and_component(case):
     """Full gradient, one unit direction per interior value: pins each boundary cell's ghost preimages."""
     p, U, dt, lam_new = _setup(case, seed=11)

@@ -35,7 +35,7 @@ def _ptr(t):
 
 
 def _gpu_ctx(p: Problem):
-    grid = H.Grid(p.ndim, p.n, 2, "Cartesian", (0.0, 0.0, 0.0), p.xmax)
+    grid = H.Grid(p.ndim, p.n, 2, p.geom, p.xmin, p.xmax)
     ctx = grid.context(H.PerfectGas(p.gamma, 1.0), H.UniformGravity(p.g), p.bc, None, 0, p.limiter, p.riemann, "fp64")
     return grid, ctx
 
@@ -95,10 +95,9 @@ def test_reverse_step_rejects_unsupported_configs(cuda):
     L = _lib.lib()
     z = torch.zeros(1, dtype=torch.float64, device="cuda")
     args = [_ptr(z)] * 10
-    g = H.Grid(2, [8, 8], 2, "Spherical", (0.1, 0.1, 0.0), (1.0, 3.0, 1.0))
-    ctx = g.context(H.PerfectGas(1.4, 1.0), None, ["Reflexive"] * 4, None, 0, "Minmod", "HLL", "fp64")
-    assert L.hclpp_step_adjoint(ctx.handle, 0.1, *args) == _lib.HCLPP_ERR_UNSUPPORTED
     g = H.Grid(2, [8, 8], 2)
+    ctx = g.context(H.PerfectGas(1.4, 1.0), None, None, None, 1, "Minmod", "HLL", "fp64")  # a passive scalar
+    assert L.hclpp_step_adjoint(ctx.handle, 0.1, *args) == _lib.HCLPP_ERR_UNSUPPORTED
     ctx = g.context(H.PerfectGas(1.4, 1.0), None, None, None, 0, "Minmod", "Low Mach", "fp64")
     assert L.hclpp_step_adjoint(ctx.handle, 0.1, *args) == _lib.HCLPP_ERR_UNSUPPORTED
     ctx = g.context(H.PerfectGas(1.4, 1.0), None, None, None, 0, "Minmod", "HLL", "fp64")
@@ -204,6 +203,50 @@ def test_rollout_gradient_matches_finite_differences(cuda):
     fdg = _fd_loss_diff(spec, rho0, u, P, gamma.detach(), dts, target, (torch.zeros_like(rho0), 1.0), h, target_E)
     relg = abs(fdg - g_gam) / max(abs(fdg), 1e-30)
     print(f"[adjoint-fd] gamma fd={fdg:.10e} adjoint={g_gam:.10e} rel={relg:.2e}", flush=True)
+    assert worst <= 1e-4
+    assert relg <= 1e-4
+
+
+def test_spherical_rollout_gradient_matches_finite_differences(cuda):
+    """2D spherical (r, theta) rollout: the geometric sources of the predictor and of the update are
+    differentiated; dL/drho0 along seeded unit directions and dL/dgamma against central differences.
+    A random unit direction sees about |grad| / sqrt(N) of the gradient (N = 288 cells here), so each
+    difference is measured against the larger of |fd| and |grad| / 10: a direction whose derivative
+    happens to be small does not turn the finite difference's own error into a large ratio."""
+    nr, nth, steps = 24, 12, 20
+    lo, hi = (0.5, 0.3, 0.0), (1.5, np.pi - 0.3, 1.0)
+    bc = ["Reflexive", "NullGradient", "Reflexive", "Reflexive"]
+    grid = H.Grid(2, [nr, nth], 2, "Spherical", lo, hi)
+    spec = RolloutSpec(grid, "VanLeer", "HLLC", bc)
+    R, TH = np.meshgrid(np.linspace(0, 1, nr), np.linspace(0, 1, nth))
+    R, TH = R[None], TH[None]  # interior arrays are [z, y, x], z of extent 1
+    rho0 = _t(1.0 + 0.2 * np.sin(2 * np.pi * R) * np.cos(np.pi * TH))
+    u = _t(np.stack([0.1 + 0.05 * np.cos(2 * np.pi * TH), 0.05 * np.sin(2 * np.pi * R)]))
+    P = _t(1.0 + 0.1 * np.cos(2 * np.pi * (R + TH)))
+    gamma = torch.tensor(1.4, dtype=torch.float64, requires_grad=True)
+    r, m, E = _cons0(rho0, u, P, gamma.detach())
+    dts = cfl_schedule(spec, r, m, E, 1.4, steps, 0.4)
+    with torch.no_grad():
+        rt, mt, et = _cons0(rho0 + 0.02 * _t(np.cos(np.pi * (R + 2 * TH))), u, P, gamma.detach())
+        target, _, target_E = rollout(rt, mt, et, gamma.detach(), spec, dts)
+    x = rho0.clone().requires_grad_(True)
+    L, _ = _loss(spec, x, u, P, gamma, dts, target, every=5, target_E=target_E)
+    L.backward()
+    g_rho, g_gam = x.grad, float(gamma.grad)
+    gnorm = float(torch.linalg.norm(g_rho))
+    rng = np.random.default_rng(1)
+    worst = 0.0
+    for k in range(4):
+        v = _t(rng.standard_normal((1, nth, nr)))
+        v = v / torch.linalg.norm(v)
+        fd = _fd_loss_diff(spec, rho0, u, P, gamma.detach(), dts, target, (v, 0.0), 1e-6, target_E)
+        ad = float(torch.sum(g_rho * v))
+        rel = abs(fd - ad) / max(abs(fd), 0.1 * gnorm, 1e-30)
+        worst = max(worst, rel)
+        print(f"[adjoint-fd spherical] dir={k} fd={fd:.10e} adjoint={ad:.10e} rel={rel:.2e}", flush=True)
+    fdg = _fd_loss_diff(spec, rho0, u, P, gamma.detach(), dts, target, (torch.zeros_like(rho0), 1.0), 1e-6, target_E)
+    relg = abs(fdg - g_gam) / max(abs(fdg), 1e-30)
+    print(f"[adjoint-fd spherical] gamma fd={fdg:.10e} adjoint={g_gam:.10e} rel={relg:.2e}", flush=True)
     assert worst <= 1e-4
     assert relg <= 1e-4
 
