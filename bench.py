@@ -144,15 +144,17 @@ def _physical_cores():
 
 
 def adjoint_block(n: int, steps: int, every: int, fd_dirs: int):
-    """Differentiable-Euler workload: forward rollout, checkpointed backward and reverse-kernel cost per step,
-    plus a central-difference check of the gradient on a few seeded unit directions."""
+    """Differentiable-Euler workload: forward rollout, backward under the default schedule (every state
+    kept when the rollout fits in half the free device memory) and under periodic checkpoints, reverse-kernel
+    cost per step, plus a central-difference check of the gradient on a few seeded unit directions."""
     import numpy as np
     import torch
 
-    from heraclespp_amd.adjoint import diff_euler_problem
+    from heraclespp_amd.adjoint import diff_euler_problem, pick_schedule, state_bytes
 
     prob = diff_euler_problem(n, steps, seed=0)
     prob.gradient(every=every)  # warm-up: allocations, first launches
+    prob.gradient(every=every, schedule="periodic")
     torch.cuda.synchronize()
 
     def timed(fn):
@@ -165,10 +167,17 @@ def adjoint_block(n: int, steps: int, every: int, fd_dirs: int):
     g = torch.tensor(prob.gamma, dtype=torch.float64)
     with torch.no_grad():
         t_fwd, _ = timed(lambda: prob.loss(prob.rho0, g, every))
-    x = prob.rho0.clone().requires_grad_(True)
-    gg = g.clone().requires_grad_(True)
-    t_fwd_ckpt, L = timed(lambda: prob.loss(x, gg, every))
-    t_bwd, _ = timed(lambda: L.backward())
+    schedule = pick_schedule(prob.spec, steps)
+    runs = {}
+    for sched in dict.fromkeys([schedule, "periodic"]):
+        x = prob.rho0.clone().requires_grad_(True)
+        gg = g.clone().requires_grad_(True)
+        t_f, L = timed(lambda: prob.loss(x, gg, every, schedule=sched))
+        t_b, _ = timed(lambda: L.backward())
+        runs[sched] = (t_f, t_b, x.grad, gg.grad)
+    t_fwd_kept, t_bwd, xgrad, _ = runs[schedule]
+    t_fwd_ckpt, t_bwd_ckpt, xgrad_ckpt, _ = runs["periodic"]
+    same = bool(torch.equal(xgrad, xgrad_ckpt)) and bool(torch.equal(runs[schedule][3], runs["periodic"][3]))
     # reverse kernels alone: `steps` reverse steps on one state
     eng = prob.spec.engine(prob.gamma)
     U = eng.load(*prob.cons(prob.rho0, g))
@@ -189,7 +198,7 @@ def adjoint_block(n: int, steps: int, every: int, fd_dirs: int):
         v = torch.from_numpy(rng.standard_normal((n, n))).cuda()
         v = v / torch.linalg.norm(v)
         fd = prob.fd(v, 0.0, 1e-6)
-        ad = float(torch.sum(x.grad * v))
+        ad = float(torch.sum(xgrad * v))
         worst = max(worst, abs(fd - ad) / max(abs(fd), 1e-300))
     # dL/dgamma of the density loss vanishes (uniform u and P: the density is a pure contact, which HLLC
     # carries independently of the wave speeds); check the gamma path on the loss with an energy term
@@ -198,14 +207,22 @@ def adjoint_block(n: int, steps: int, every: int, fd_dirs: int):
     fdg = probE.fd(torch.zeros_like(probE.rho0), 1.0, 1e-6)
     relg = abs(fdg - gE) / max(abs(fdg), 1e-300)
     f = t_fwd / steps * 1e3
+    kept = {"store_all": steps + 1, "periodic": len(range(0, steps, every)) + min(every, steps)}[schedule]
     return {
-        "workload": f"differentiable Euler {n}x{n} periodic, VanAlbada+HLLC, {steps} steps, checkpoint every {every}",
+        "workload": f"differentiable Euler {n}x{n} periodic, VanAlbada+HLLC, {steps} steps",
+        "backward_schedule": schedule if schedule == "store_all" else f"periodic, checkpoint every {every}",
+        "backward_states_GB": round(kept * state_bytes(prob.spec) / 1e9, 2),
         "forward_ms_per_step": round(f, 5),
-        "forward_with_checkpoints_ms_per_step": round(t_fwd_ckpt / steps * 1e3, 5),
+        "forward_keeping_states_ms_per_step": round(t_fwd_kept / steps * 1e3, 5),
         "backward_ms_per_step": round(t_bwd / steps * 1e3, 5),
         "reverse_kernels_ms_per_step": round(t_rev / steps * 1e3, 5),
         "ratio_reverse_to_forward": round(t_rev / t_fwd, 3),
         "ratio_backward_to_forward": round(t_bwd / t_fwd, 3),
+        "checkpointed_every": every,
+        "checkpointed_forward_ms_per_step": round(t_fwd_ckpt / steps * 1e3, 5),
+        "checkpointed_backward_ms_per_step": round(t_bwd_ckpt / steps * 1e3, 5),
+        "ratio_checkpointed_backward_to_forward": round(t_bwd_ckpt / t_fwd, 3),
+        "gradients_bitwise_equal_across_schedules": same,
         "fd_directions": fd_dirs,
         "fd_rel_err_max": float(f"{worst:.3e}"),
         "fd_rel_err_gamma": float(f"{relg:.3e}"),

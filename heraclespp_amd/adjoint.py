@@ -4,12 +4,17 @@ whose backward runs the hand-written reverse-mode step kernels (hclpp_step_adjoi
 * dt is held fixed: a rollout replays a dt schedule (by default the CFL schedule of the
   unperturbed forward, recorded on the device by hclpp_clock_record_dt), so the function
   being differentiated is U_N = S_{dt_N} o ... o S_{dt_1}(U_0; gamma).
-* Memory stays bounded by checkpointing: the forward keeps the state every
-  `checkpoint_every` steps; the backward walks the segments from last to first,
-  recomputes each segment's states from its checkpoint with the same fused forward
-  kernel (bit-identical to the original forward) and runs the reverse steps over it.
-  `schedule="binomial"` instead keeps at most `snapshots` states alive and places them by
-  the binomial (revolve) rule, trading more recomputation for O(snapshots) memory.
+* The schedule trades memory for recomputation.  `schedule="store_all"` keeps every state
+  of the forward (N + 1 ghost-inclusive states, written in place by the fused steps) and the
+  backward reverses all N steps in one call, with no recomputation.  `"periodic"` keeps the
+  state every `checkpoint_every` steps; the backward walks the segments from last to first,
+  recomputes each segment's states from its checkpoint with the same fused forward kernel
+  (bit-identical to the original forward) and runs the reverse steps over it.  `"binomial"`
+  keeps at most `snapshots` states alive and places them by the binomial (revolve) rule,
+  trading more recomputation for O(snapshots) memory.  `"auto"` (the default) takes
+  store_all when the N + 1 states fit in `memory_budget` bytes (by default half the device
+  memory free at the call), else periodic.  Every schedule gives bit-identical gradients.
+* A rollout whose inputs need no gradient (or that runs with grad disabled) keeps no state.
 * Inputs and outputs are interior arrays (ghost cells are filled by the boundary
   conditions inside the rollout, so they carry no gradient).
 * gamma is a differentiable input: its gradient is the sum over steps of the reverse
@@ -91,6 +96,15 @@ class StepEngine:
     def new_state(self):
         g, nd = self.spec.grid, self.nd
         return (g.zeros(), g.zeros(nd), g.zeros())
+
+    def new_states(self, n: int):
+        """n states as views of one uninitialised allocation per variable, for step outputs: a step
+        writes every cell of its output (the interior, then the whole ghost shell)."""
+        shape, nd = self.spec.grid.shape3, self.nd
+        rho = torch.empty((n,) + shape, dtype=torch.float64, device="cuda")
+        rhou = torch.empty((n, nd) + shape, dtype=torch.float64, device="cuda")
+        E = torch.empty((n,) + shape, dtype=torch.float64, device="cuda")
+        return [(rho[k], rhou[k], E[k]) for k in range(n)]
 
     def load(self, rho, rhou, E, out=None):
         """Interior arrays -> ghost-inclusive state with its ghosts filled."""
@@ -208,13 +222,16 @@ class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rho, rhou, E, gamma, spec, dts, every, schedule, snapshots, overlap):
         eng = spec.engine(float(gamma))
-        U0 = eng.load(rho, rhou, E)
         N = len(dts)
-        if schedule == "periodic":
-            keep = set(range(0, N, every))
+        if schedule == "store_all":
+            states = [eng.load(rho, rhou, E)] + eng.new_states(N)
+            eng.start(states[0])
+            eng.steps(states, dts)
+            eng.check_clock(N)
+            kept, last = dict(enumerate(states)), states[N]
         else:
-            keep = {0}
-        kept, last = forward_states(eng, U0, dts, keep)
+            keep = {"periodic": set(range(0, N, every)), "binomial": {0}, "none": set()}[schedule]
+            kept, last = forward_states(eng, eng.load(rho, rhou, E), dts, keep)
         ctx.spec, ctx.dts, ctx.every, ctx.schedule, ctx.snapshots = spec, list(dts), every, schedule, snapshots
         ctx.overlap = overlap
         ctx.kept = kept
@@ -245,8 +262,12 @@ class _Rollout(torch.autograd.Function):
             _reverse_segment(eng, seg, s, e, ctx.dts, dgam, state)
 
         N = len(ctx.dts)
+        if ctx.schedule == "none":
+            raise RuntimeError("the rollout ran without grad and kept no state to reverse")
         eng.start(ctx.kept[0])  # the clock only gates the recomputation steps here; dt comes from the schedule
-        if ctx.schedule == "periodic" and ctx.overlap and len(ctx.kept) > 1:
+        if ctx.schedule == "store_all":
+            _reverse_segment(eng, [ctx.kept[k] for k in range(N)], 0, N, ctx.dts, dgam, state)
+        elif ctx.schedule == "periodic" and ctx.overlap and len(ctx.kept) > 1:
             side = spec.side_engine(ctx.gamma_value)
             side.start(ctx.kept[0])
             _reverse_periodic_overlapped(eng, side, ctx.kept, ctx.dts, ctx.every, dgam, state)
@@ -365,21 +386,44 @@ def _reverse_binomial(eng, U, first, last, dts, snaps, reverse, state):
     _reverse_binomial(eng, U, first, first + m, dts, snaps, reverse, state)
 
 
-def rollout(rho, rhou, E, gamma, spec: RolloutSpec, dts, checkpoint_every: int = 20, schedule: str = "periodic",
-            snapshots: int = 8, overlap: bool = True):
+def state_bytes(spec: RolloutSpec) -> int:
+    """Bytes of one ghost-inclusive FP64 state (rho, rhou, E) of the spec's grid."""
+    g = spec.grid
+    cells = g.Nx_local_wg[0] * g.Nx_local_wg[1] * g.Nx_local_wg[2]
+    return cells * (g.ndim + 2) * 8
+
+
+def pick_schedule(spec: RolloutSpec, nsteps: int, memory_budget: int | None = None) -> str:
+    """"store_all" when the nsteps + 1 states of a rollout fit in memory_budget bytes (default: half the
+    device memory free now), else "periodic"."""
+    if memory_budget is None:
+        memory_budget = torch.cuda.mem_get_info()[0] // 2
+    return "store_all" if (nsteps + 1) * state_bytes(spec) <= memory_budget else "periodic"
+
+
+def rollout(rho, rhou, E, gamma, spec: RolloutSpec, dts, checkpoint_every: int = 20, schedule: str = "auto",
+            snapshots: int = 8, overlap: bool = True, memory_budget: int | None = None):
     """Differentiable N-step rollout of interior arrays (rho, rhou, E) with the fixed dt schedule `dts`.
 
-    gamma: a 0-d float64 tensor (may require grad).  schedule: "periodic" (a checkpoint every
-    `checkpoint_every` steps, segment recomputation) or "binomial" (at most `snapshots` live
-    states, revolve placement).  overlap (periodic schedule): recompute the next segment on a
-    second stream while the current one is reversed (same results, bit for bit)."""
-    if schedule not in ("periodic", "binomial"):
-        raise ValueError("schedule must be 'periodic' or 'binomial'")
+    gamma: a 0-d float64 tensor (may require grad).  schedule: "store_all" (every state kept, no
+    recomputation), "periodic" (a checkpoint every `checkpoint_every` steps, segment
+    recomputation), "binomial" (at most `snapshots` live states, revolve placement) or "auto"
+    (store_all if the N + 1 states fit in `memory_budget` bytes, default half the free device
+    memory; else periodic).  overlap (periodic schedule): recompute the next segment on a second
+    stream while the current one is reversed (same results, bit for bit).  With grad disabled, or
+    no input that requires grad, no state is kept."""
+    if schedule not in ("auto", "store_all", "periodic", "binomial"):
+        raise ValueError("schedule must be 'auto', 'store_all', 'periodic' or 'binomial'")
     if checkpoint_every < 1:
         raise ValueError("checkpoint_every must be >= 1")
     if not torch.is_tensor(gamma):
         gamma = torch.tensor(float(gamma), dtype=torch.float64)
-    return _Rollout.apply(rho, rhou, E, gamma, spec, list(dts), checkpoint_every, schedule, snapshots, overlap)
+    dts = list(dts)
+    if not (torch.is_grad_enabled() and any(t.requires_grad for t in (rho, rhou, E, gamma))):
+        schedule = "none"
+    elif schedule == "auto":
+        schedule = pick_schedule(spec, len(dts), memory_budget)
+    return _Rollout.apply(rho, rhou, E, gamma, spec, dts, checkpoint_every, schedule, snapshots, overlap)
 
 
 class DiffEulerProblem:
@@ -426,7 +470,7 @@ class DiffEulerProblem:
         """E = P / (gamma - 1) + rho |u|^2 / 2, m = rho u."""
         return rho0, rho0.unsqueeze(0) * self.u, self.P / (gamma - 1) + 0.5 * rho0 * (self.u * self.u).sum(0)
 
-    def loss(self, rho0, gamma, every=20, schedule="periodic", snapshots=8):
+    def loss(self, rho0, gamma, every=20, schedule="auto", snapshots=8):
         r, m, E = self.cons(rho0, gamma)
         rT, _, ET = rollout(r, m, E, gamma, self.spec, self.dts, checkpoint_every=every, schedule=schedule,
                             snapshots=snapshots)
@@ -435,7 +479,7 @@ class DiffEulerProblem:
             L = L + 0.5 * self.energy_weight * torch.sum((ET - self.target_E) ** 2) * self.dV
         return L
 
-    def gradient(self, every=20, schedule="periodic", snapshots=8):
+    def gradient(self, every=20, schedule="auto", snapshots=8):
         x = self.rho0.clone().requires_grad_(True)
         g = torch.tensor(self.gamma, dtype=torch.float64, requires_grad=True)
         L = self.loss(x, g, every, schedule, snapshots)

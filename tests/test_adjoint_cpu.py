@@ -217,3 +217,36 @@ def test_batched_segments_chain_the_adjoint_buffers(N, every):
     assert float(state["lam"][0]) == want
     assert state["reverse_steps"] == N and float(dgam) == N
     assert state["forward_steps"] == N - len(kept)
+
+
+@pytest.mark.parametrize("N", [1, 2, 7, 40])
+def test_store_all_reverse_is_one_segment_over_every_kept_state(N):
+    """The store-all backward: all N states kept, one reverse call over them, no recomputation."""
+    from heraclespp_amd.adjoint import _reverse_segment
+
+    torch = pytest.importorskip("torch")
+    eng = _BatchedEngine()
+    states = [(torch.zeros(1, dtype=torch.float64),)] + [eng.new_state() for _ in range(N)]
+    eng.steps(states, [0.1] * N)
+    dgam = torch.zeros(1, dtype=torch.float64)
+    state = {"lam": (torch.full((1,), 1.0, dtype=torch.float64),), "lam2": (torch.zeros(1, dtype=torch.float64),),
+             "reverse_steps": 0, "forward_steps": 0}
+    _reverse_segment(eng, states[:N], 0, N, [0.1] * N, dgam, state)
+    want = 1.0
+    for k in range(N - 1, -1, -1):
+        want = 2 * want + k
+    assert float(state["lam"][0]) == want
+    assert state["reverse_steps"] == N and float(dgam) == N and state["forward_steps"] == 0
+
+
+def test_schedule_pick_follows_the_memory_budget():
+    from heraclespp_amd import hydro as H
+    from heraclespp_amd.adjoint import RolloutSpec, pick_schedule, state_bytes
+
+    spec = RolloutSpec(H.Grid(2, [32, 16, 1], 2), "VanLeer", "HLLC", ["Periodic"] * 4)
+    one = state_bytes(spec)
+    assert one == 36 * 20 * 4 * 8  # ghost-inclusive cells x (rho, 2 momenta, E) x 8 B
+    assert pick_schedule(spec, 10, 11 * one) == "store_all"
+    assert pick_schedule(spec, 10, 11 * one - 1) == "periodic"
+    spec3 = RolloutSpec(H.Grid(3, [8, 6, 4], 2), "Minmod", "HLL", ["Periodic"] * 6)
+    assert state_bytes(spec3) == 12 * 10 * 8 * 5 * 8

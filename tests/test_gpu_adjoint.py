@@ -259,9 +259,11 @@ def test_checkpoint_schedules_agree_bitwise(cuda):
     target = torch.ones_like(rho0)
     grads = []
     # periodic with the next segment recomputed on a second stream during each reversal (overlap),
-    # and without; single-segment periodic; binomial with 3 and 1 snapshots
+    # and without; single-segment periodic; binomial with 3 and 1 snapshots; every state kept (one
+    # reverse call over all steps), chosen explicitly and by the memory budget
     for schedule, every, snaps, overlap in (("periodic", 5, 0, True), ("periodic", 5, 0, False), ("periodic", 4, 0, True),
-                                            ("periodic", 23, 0, True), ("binomial", 1, 3, False), ("binomial", 1, 1, False)):
+                                            ("periodic", 23, 0, True), ("binomial", 1, 3, False), ("binomial", 1, 1, False),
+                                            ("store_all", 5, 0, False), ("auto", 5, 0, True)):
         x = rho0.clone().requires_grad_(True)
         gamma = gamma0.clone().requires_grad_(True)
         L, _ = _loss(spec, x, u, P, gamma, dts, target, every=every, schedule=schedule, snapshots=snaps, overlap=overlap)
@@ -270,6 +272,33 @@ def test_checkpoint_schedules_agree_bitwise(cuda):
     for g in grads[1:]:
         assert torch.equal(g[0], grads[0][0])
         assert torch.equal(g[1], grads[0][1])
+
+
+def test_step_writes_every_cell_of_its_output(cuda):
+    """The store-all schedule keeps step outputs in uninitialised memory: a step must write the whole
+    ghost-inclusive output.  Outputs poisoned with NaN equal outputs that start at zero, bit for bit."""
+    from heraclespp_amd.adjoint import StepEngine
+
+    for bc in (None, ["Periodic", "Periodic", "NullGradient", "Reflexive"]):
+        s, spec, rho0, u, P = _smooth(24, 3)
+        if bc is not None:
+            spec = RolloutSpec(spec.grid, s.reconstruction, s.riemann_solver, bc)
+        gamma = torch.tensor(1.4, dtype=torch.float64)
+        r, m, E = _cons0(rho0, u, P, gamma)
+        eng = StepEngine(spec, 1.4)
+        dts = [1e-3] * 3
+        outs = []
+        for poison in (0.0, float("nan")):
+            states = [eng.load(r, m, E)] + eng.new_states(3)
+            for U in states[1:]:
+                for t in U:
+                    t.fill_(poison)
+            eng.start(states[0])
+            eng.steps(states, dts)
+            eng.check_clock(3)
+            outs.append([t.clone() for U in states[1:] for t in U])
+        for a, b in zip(*outs):
+            assert torch.equal(a, b)
 
 
 def test_batched_reverse_steps_equal_single_steps(cuda):
@@ -334,7 +363,8 @@ def test_batched_reverse_steps_equal_single_steps(cuda):
 
 def test_diff_euler_1024_gradient_64_directions(cuda):
     """The differentiable-Euler configuration at full size: 1024^2 periodic, VanAlbada + HLLC,
-    400 steps with a checkpoint every 20; loss 1/2 sum (rho_T - rho_tgt)^2 dV with the target the
+    400 steps under the default schedule (every state kept; the periodic schedule gives the same
+    gradient bit for bit, test_checkpoint_schedules_agree_bitwise); loss 1/2 sum (rho_T - rho_tgt)^2 dV with the target the
     forward of a seeded perturbation of rho0; dL/drho0 along 64 seeded unit directions against
     central differences with h = 1e-6 (relative agreement <= 1e-4).
 
