@@ -128,10 +128,17 @@ template <int ND, int LIM>
 __global__ __launch_bounds__(256) void k_adj_gather(GridDev G, Range3 rg, Range3 interior, PerfectGas eos, BcSpec bc, AdjArrays A, NextState nx,
                                                     double* __restrict__ partials)
 {
+    // a workgroup whose cells all lie where no ghost copies from (adj::no_ghost_copies) takes the
+    // gather without the preimage search; the test is uniform over the workgroup
+    int const i0 = rg.b[0] + (int)blockIdx.x * kBX, j0 = rg.b[1] + (int)blockIdx.y * kBY;
+    bool deep = adj::no_ghost_copies(G, 0, i0, i0 + kBX);
+    if (ND >= 2) deep = deep && adj::no_ghost_copies(G, 1, j0, j0 + kBY);
+    if (ND == 3) deep = deep && adj::no_ghost_copies(G, 2, rg.b[2] + (int)blockIdx.z, rg.b[2] + (int)blockIdx.z + 1);
     int i, j, k;
     double lg = 0;
     if (range_cell(rg, i, j, k)) {
-        if (in_range(interior, i, j, k)) lg = adj::cell_gather<ND, LIM>(G, eos, bc, A, i, j, k);
+        if (deep) lg = adj::cell_gather<ND, LIM, true>(G, eos, bc, A, i, j, k);
+        else if (in_range(interior, i, j, k)) lg = adj::cell_gather<ND, LIM>(G, eos, bc, A, i, j, k);
         if (nx.rho) {
             AdjArrays B = A;
             B.rho = nx.rho;

@@ -464,23 +464,25 @@ __host__ __device__ __forceinline__ double cell_hancock(const GridDev& G, const 
 
 // Adjoint of the primitive state of cell c (any cell of the array): its own part plus the
 // shares of its +-1 neighbours along each dim (zero outside the ring-1 cells).
-template <int ND>
+// INTERIOR: c is an interior cell, so c and its +-1 neighbours are ring-1 cells and every test
+// below holds (the same sums in the same order, without the tests).
+template <int ND, bool INTERIOR = false>
 __host__ __device__ __forceinline__ S<ND> cell_prim_adjoint(const GridDev& G, const AdjArrays& A, const int* c)
 {
     constexpr int NV = ND + 2;
     int64_t const pl = G.plane;
-    S<ND> lp = ring1<ND>(G, c) ? load_planes<ND>(A.face, flat(G, c[0], c[1], c[2]), pl, 0) : zero<ND>();
+    S<ND> lp = (INTERIOR || ring1<ND>(G, c)) ? load_planes<ND>(A.face, flat(G, c[0], c[1], c[2]), pl, 0) : zero<ND>();
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         int q[3] = {c[0], c[1], c[2]};
         q[d] = c[d] - 1;
-        if (ring1<ND>(G, q)) {
+        if (INTERIOR || ring1<ND>(G, q)) {
             S<ND> const ra = load_planes<ND>(A.face, flat(G, q[0], q[1], q[2]), pl, share_up(NV, d));
 #pragma unroll
             for (int v = 0; v < NV; ++v) lp[v] += ra[v];
         }
         q[d] = c[d] + 1;
-        if (ring1<ND>(G, q)) {
+        if (INTERIOR || ring1<ND>(G, q)) {
             S<ND> const rb = load_planes<ND>(A.face, flat(G, q[0], q[1], q[2]), pl, share_down(NV, ND, d));
 #pragma unroll
             for (int v = 0; v < NV; ++v) lp[v] -= rb[v];
@@ -516,8 +518,19 @@ __host__ __device__ __forceinline__ int preimages(const GridDev& G, const BcSpec
     return m;
 }
 
+// Interior cells from which no ghost cell is filled, along dim d: [2 ng, n) in ghost-inclusive
+// indices (a periodic wrap fills the ghosts from [ng, 2 ng) and [n, n + ng), a Reflexive mirror
+// from the same cells, a NullGradient clamp from ng and ng + n - 1).
+__host__ __device__ __forceinline__ bool no_ghost_copies(const GridDev& G, int d, int lo, int hi)
+{
+    return lo >= 2 * G.ngh[d] && hi <= G.n[d];
+}
+
 // ---------------------------------------------------------------- pass 5: gather (interior cells)
-template <int ND, int LIM>
+// DEEP: the cell is interior and no ghost is filled from it along any dim (no_ghost_copies), so
+// its only preimage is itself: the same sums in the same order as the general case, without the
+// preimage search.
+template <int ND, int LIM, bool DEEP = false>
 __host__ __device__ __forceinline__ double cell_gather(const GridDev& G, const PerfectGas& eos, const BcSpec& bc, const AdjArrays& A, int i,
                                                        int j, int k)
 {
@@ -533,6 +546,21 @@ __host__ __device__ __forceinline__ double cell_gather(const GridDev& G, const P
         double const g = gravity(G, i, d);
         out[0] += A.dt * g * ln[1 + d];
         out[1 + d] += A.dt * g * ln[ND + 1];
+    }
+    if constexpr (DEEP) {
+        double lg = 0;
+        S<ND> const lp = cell_prim_adjoint<ND, true>(G, A, c);
+        S<ND> lu = zero<ND>();
+        prim_vjp(load_cons<ND>(A.rho, A.rhou, A.E, id, pl), eos.gamma, lp, lu, lg);
+        out[0] += lu[0];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) out[1 + d] += lu[1 + d];
+        out[ND + 1] += lu[ND + 1];
+        A.lrho[id] = out[0];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) A.lrhou[id + pl * d] = out[1 + d];
+        A.lE[id] = out[ND + 1];
+        return lg;
     }
     int pre[3][kMaxPre];
     bool flip[3][kMaxPre];

@@ -64,6 +64,29 @@ def test_gpu_reverse_step_matches_host_build(cuda, case):
     assert abs(gpu[3] - host[3]) <= 1e-12 * max(1.0, abs(host[3]))
 
 
+# Grids wide enough for the gather pass's workgroups (64 x 4 cells) to lie where no ghost copies
+# from, which take the gather without the preimage search; the host build searches every cell.
+LARGE_CASES = [
+    ("2d_large_vanalbada_hllc_mixed_g", 2, [160, 24], "VanAlbada", "HLLC", ["Periodic", "Periodic", "Reflexive", "NullGradient"],
+     (0.0, -0.2, 0)),
+    ("3d_large_minmod_hll_periodic_g", 3, [136, 12, 10], "Minmod", "HLL", ["Periodic"] * 6, (0, 0, -0.1)),
+    ("3d_large_vanleer_hllc_mixed", 3, [140, 13, 11], "VanLeer", "HLLC",
+     ["Periodic", "Periodic", "Reflexive", "NullGradient", "Reflexive", "Reflexive"], (0, 0, 0)),
+]
+
+
+@pytest.mark.parametrize("case", LARGE_CASES, ids=[c[0] for c in LARGE_CASES])
+def test_gpu_reverse_step_matches_host_build_on_wide_grids(cuda, case):
+    print(f"[adjoint] case={case[0]}", flush=True)
+    p, U, dt, lam_new = _setup(case, seed=6)
+    host = p.host_adjoint(*U, *lam_new, dt)
+    gpu = _gpu_adjoint(p, U, lam_new, dt)
+    it = p.interior()
+    for a, b in zip(gpu[:3], host[:3]):
+        np.testing.assert_array_equal(a[(Ellipsis,) + it], b[(Ellipsis,) + it])
+    assert abs(gpu[3] - host[3]) <= 1e-12 * max(1.0, abs(host[3]))
+
+
 def test_gpu_reverse_step_is_deterministic(cuda):
     p, U, dt, lam_new = _setup(CASES[6], seed=4)
     a = _gpu_adjoint(p, U, lam_new, dt)
