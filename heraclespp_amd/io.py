@@ -9,6 +9,9 @@ fx0..fx<nfx-1>, shape [nz, ny, nx].
 
 The device state is copied to the host before writing, as write_pdi does through host views; the
 temperature is computed on the GPU first (analysis/temperature.hpp, hclpp_temperature).
+
+write_xdmf writes the XDMF side file <directory>/<prefix>.xmf that indexes the outputs of a run
+(XmlWriter, io.cpp:433-566), for viewers that read XDMF.
 """
 
 from __future__ import annotations
@@ -178,6 +181,76 @@ def temperature(grid, eos, rho, P):
     _lib.check(_lib.lib().hclpp_temperature(ctx.handle, ctypes.byref(r), ctx._chk(rho_d, (), "rho"), ctx._chk(P_d, (), "P"),
                                             ctx._chk(T, (), "T")))
     return T
+
+
+def xdmf_variables(ndim: int, nfx: int) -> list:
+    """Cell variables the XDMF file lists, in the reference's order (io.cpp:433-445): rho, P, E, T,
+    the passive scalars, then the velocity components."""
+    return ["rho", "P", "E", "T"] + [f"fx{f}" for f in range(nfx)] + [f"ux{d}" for d in range(ndim)]
+
+
+def write_xdmf(directory: str, prefix: str, n, ndim: int, nfx: int, output_id: int, outputs_record) -> str:
+    """The XDMF side file <directory>/<prefix>.xmf of XmlWriter (io.cpp:447-566): a temporal collection
+    with one rectilinear grid per recorded output, whose node coordinates are the x0_ng, x1_ng and
+    x2_ng datasets and whose cell attributes are the datasets of its HDF5 file.  outputs_record holds
+    (iter, t) of the outputs written since the start of the run (restart included), the last one being
+    output_id; the file is rewritten whole at every output, as the reference does (main.cpp:313-322,
+    :533-538).  n: interior cells per dim (3 values)."""
+    n = [int(v) for v in (list(n) + [1, 1, 1])[:3]]
+    lines = []
+    level = 0
+
+    def put(text):
+        lines.append("  " * level + text)
+
+    def item(dims, ref):
+        nonlocal level
+        put(f'<DataItem NumberType="Float" Precision="8" Dimensions="{dims}" Format="HDF">')
+        level += 1
+        put(ref)
+        level -= 1
+        put("</DataItem>")
+
+    put('<?xml version="1.0"?>')
+    put('<!DOCTYPE Xdmf SYSTEM "Xdmf.dtd" []>')
+    put('<Xdmf Version="2.0">')
+    level = 1
+    put("<Domain>")
+    level = 2
+    put('<Grid Name="TimeSeries" GridType="Collection" CollectionType="Temporal">')
+    first = output_id + 1 - len(outputs_record)
+    for i, (_, t) in enumerate(outputs_record):
+        fname = output_filename(prefix, first + i)
+        level = 3
+        put('<Grid Name="output" GridType="Uniform">')
+        level = 4
+        put(f'<Time Value="{t:g}"/>')
+        put(f'<Topology TopologyType="3DRectMesh" Dimensions="{n[2] + 1} {n[1] + 1} {n[0] + 1}"/>')
+        put('<Geometry GeometryType="VXVYVZ">')
+        for d in range(3):
+            level = 5
+            item(n[d] + 1, f"{fname}:/x{d}_ng")
+        level = 4
+        put("</Geometry>")
+        for v in xdmf_variables(ndim, nfx):
+            put(f'<Attribute Center="Cell" Name="{v}" AttributeType="Scalar">')
+            level = 5
+            item(f"{n[2]} {n[1]} {n[0]}", f"{fname}:/{v}")
+            level = 4
+            put("</Attribute>")
+        level = 3
+        put("</Grid>")
+    level = 2
+    put("</Grid>")
+    level = 1
+    put("</Domain>")
+    level = 0
+    put("</Xdmf>")
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, prefix + ".xmf")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
 
 
 @dataclass

@@ -85,6 +85,7 @@ class HydroSolver:
         # output schedule of main.cpp:362-375 (frequencies 0: none) and the output counters
         self._sched = (0, 0.0, 0.0)
         self.output_id = self.iter_output_id = self.time_output_id = -1
+        self.outputs_record = []  # (iter, t) of the outputs of this run, for the XDMF side file
         self.restarted = False
         if not self.fused:
             self.rho_rec = z(nd, 2)
@@ -159,6 +160,7 @@ class HydroSolver:
         hdr, rho, u, P, fx, _ = read_state(self.grid, filename, self.nfx)
         self.init_from_prim(rho, u, P, fx, t0=hdr.current_time, iter0=hdr.iter)
         self.output_id, self.iter_output_id, self.time_output_id = hdr.output_id, hdr.iter_output_id, hdr.time_output_id
+        self.outputs_record = []  # main.cpp:210: a restarted run indexes its own outputs only
         self.restarted = True
         return hdr
 
@@ -166,13 +168,20 @@ class HydroSolver:
         """Output schedule of main.cpp:362-375 for the following steps (frequencies 0: none)."""
         self._sched = (int(iter_output_frequency), float(time_first_output), float(time_output_frequency))
 
-    def write_output(self, schedule) -> str:
-        """write_pdi (io.cpp:185-324) of the current state with the current output counters."""
-        from .io import Header, write_state
+    def write_output(self, schedule, record: bool = True) -> str:
+        """write_pdi (io.cpp:185-324) of the current state with the current output counters; with
+        record, the output also joins the run's XDMF side file (main.cpp:320-322, :536-538; the
+        output written before the invalid-internal-energy error, main.cpp:399-425, does not)."""
+        from .io import Header, write_state, write_xdmf
 
         hdr = Header(self.output_id, self.iter_output_id, self.time_output_id, self.iter, self.t, self.eos.adiabatic_index())
-        return write_state(self.grid, self.eos, schedule.directory, schedule.prefix, hdr, self.rho, self.u, self.P, self.E,
+        path = write_state(self.grid, self.eos, schedule.directory, schedule.prefix, hdr, self.rho, self.u, self.P, self.E,
                            self.fx if self.nfx else None)
+        if record:
+            self.outputs_record.append((self.iter, self.t))
+            write_xdmf(schedule.directory, schedule.prefix, self.grid.Nx_glob_ng, self.grid.ndim, self.nfx, self.output_id,
+                       self.outputs_record)
+        return path
 
     def simulate(self, t_end: float, max_iter: int, schedule=None, t_ini: float = 0.0) -> list:
         """The time loop of main.cpp:313-557 with outputs: an initial output when a frequency is set and
@@ -196,7 +205,7 @@ class HydroSolver:
                     done, make_output = self._step_operators(t_end, max_iter, outputs=True)
                 except _InvalidInternalEnergy:
                     self.output_id += 1  # main.cpp:399-425: output, then the error
-                    written.append(self.write_output(schedule))
+                    written.append(self.write_output(schedule, record=False))
                     raise
                 if make_output:
                     self.output_id += 1
@@ -231,7 +240,7 @@ class HydroSolver:
                 raise RuntimeError("Error: the time step is negative")
             if clk.status:
                 self.output_id += 1  # main.cpp:399-425
-                written.append(self.write_output(schedule))
+                written.append(self.write_output(schedule, record=False))
                 raise RuntimeError(f"Time = {clk.t}, iteration = {clk.iter}: detected invalid volumic internal energy")
             if clk.output_due:
                 self.output_id += 1

@@ -12,6 +12,8 @@
   allowed by max_iter ends the run normally, as in the reference.
 """
 
+import os
+
 import numpy as np
 import pytest
 
@@ -68,6 +70,21 @@ def _check_file(path, grid, want, s):
     assert np.array_equal(T[I], (wP[I] * 1.0 * 1.672649e-27) / (wrho[I] * 1.380649e-23)), path
 
 
+def _check_xdmf(xmf, files, grid):
+    """The run's XDMF file lists exactly its output files, in order, at their times (printed with
+    6 significant digits, as the reference's ostream does)."""
+    import xml.etree.ElementTree as ET
+
+    grids = ET.fromstring("\n".join(open(xmf).read().splitlines()[2:])).findall("Domain/Grid/Grid")
+    assert len(grids) == len(files)
+    for g, path in zip(grids, files):
+        name = os.path.basename(path)
+        assert g.find("Geometry/DataItem").text.strip() == f"{name}:/x0_ng"
+        assert g.find("Attribute/DataItem").text.strip() == f"{name}:/rho"
+        t = hio.read_state(grid, path)[0].current_time
+        assert g.find("Time").attrib["Value"] == f"{t:g}"
+
+
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "operators"])
 def test_simulate_time_outputs_match_oracle(cuda, tmp_path, fused):
     s = _kh()
@@ -80,6 +97,7 @@ def test_simulate_time_outputs_match_oracle(cuda, tmp_path, fused):
     assert len(files) == len(want) == 7  # initial, t = 0.004 k (k = 1..5), t_end
     for path, w in zip(files, want):
         _check_file(path, sol.grid, w, s)
+    _check_xdmf(tmp_path / "kh.xmf", files, sol.grid)
     assert sol.t == o.t and sol.iter == o.iter
     assert np.array_equal(sol.E.cpu().numpy(), o.state("E"))
 
@@ -113,6 +131,7 @@ def test_restart_matches_oracle_restart(cuda, tmp_path):
     assert len(files2) == len(want) == 3  # no initial output on restart: t = 0.016, 0.020, t_end
     for path, w in zip(files2, want):
         _check_file(path, sol.grid, w, s)
+    _check_xdmf(tmp_path / "restarted" / "kh.xmf", files2, sol.grid)  # the restarted run indexes its own outputs
     for name, g in (("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E)):
         got = g.cpu().numpy()
         assert np.array_equal(got, o.state(name)), name
@@ -153,3 +172,4 @@ def test_internal_energy_check_runs_on_the_state_each_iteration_starts_from(cuda
         sol.simulate(1e30, 10, sched)
     hdr = hio.read_state(sol.grid, str(tmp_path / "bad_00000000.h5"))[0]
     assert hdr.iter == 0 and hdr.output_id == 0 and sol.iter == 0 and sol.t == 0.0
+    assert not (tmp_path / "bad.xmf").exists()  # the output before the error joins no XDMF file

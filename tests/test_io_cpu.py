@@ -119,6 +119,69 @@ def test_read_rejects_another_grid(tmp_path):
         hio.read_state(g, path, nfx=1)
 
 
+# ---- the XDMF side file (io.cpp:433-566)
+def _h5_dims(path):
+    head = _h5dump("-H", path)
+    out = {}
+    for name, space in re.findall(r'DATASET "(\w+)" \{\s*DATATYPE\s+\S+\s*DATASPACE\s+(SCALAR|SIMPLE \{[^}]*\})', head):
+        m = re.match(r"SIMPLE \{ \( ([^)]*) \)", space)
+        out[name] = m.group(1).replace(",", "") if m else "scalar"
+    return out
+
+
+@pytest.mark.parametrize("nd,n,nfx", [(1, (7, 1, 1), 0), (2, (6, 5, 1), 2), (3, (5, 4, 3), 1)])
+def test_xdmf_indexes_the_output_files(tmp_path, nd, n, nfx):
+    """Every data item of the XDMF file names a dataset of its output file whose shape is the item's
+    Dimensions (read by h5dump); grids are the recorded outputs, in order, with their times."""
+    import xml.etree.ElementTree as ET
+
+    if not os.path.exists(H5DUMP):
+        pytest.skip("h5dump not installed")
+    g = Grid(nd, n, 2, "Cartesian", (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    rho, u, P, E, T, fx = _state(g, nfx)
+    # a restarted run: outputs 3..5 of a run, the record holding only those (main.cpp:210)
+    record = [(40, 0.1 + 0.2), (80, 1e-7), (120, 2.5)]
+    for k, (it, t) in enumerate(record):
+        hio.write_state(g, None, str(tmp_path), "run", hio.Header(3 + k, k, k, it, t, 1.4), rho, u, P, E, fx, T=T)
+    path = hio.write_xdmf(str(tmp_path), "run", g.Nx_glob_ng, nd, nfx, 5, record)
+    assert os.path.basename(path) == "run.xmf"
+    text = open(path).read()
+    lines = text.splitlines()
+    assert lines[:3] == ['<?xml version="1.0"?>', '<!DOCTYPE Xdmf SYSTEM "Xdmf.dtd" []>', '<Xdmf Version="2.0">']
+    assert lines[3] == "  <Domain>" and lines[-1] == "</Xdmf>"
+    root = ET.fromstring("\n".join(lines[2:]))  # the DOCTYPE line is not parsed
+    series = root.find("Domain/Grid")
+    assert series.attrib == {"Name": "TimeSeries", "GridType": "Collection", "CollectionType": "Temporal"}
+    grids = series.findall("Grid")
+    assert len(grids) == 3
+    nx = [int(v) for v in (list(n) + [1, 1])[:3]]
+    want_vars = ["rho", "P", "E", "T"] + [f"fx{f}" for f in range(nfx)] + [f"ux{d}" for d in range(nd)]
+    for k, grid in enumerate(grids):
+        fname = f"run_{3 + k:08d}.h5"
+        dims = _h5_dims(str(tmp_path / fname))
+        assert grid.find("Time").attrib["Value"] == ("0.3", "1e-07", "2.5")[k]  # C++ ostream default: 6 digits
+        topo = grid.find("Topology").attrib
+        assert topo == {"TopologyType": "3DRectMesh", "Dimensions": f"{nx[2] + 1} {nx[1] + 1} {nx[0] + 1}"}
+        geo = grid.find("Geometry")
+        assert geo.attrib == {"GeometryType": "VXVYVZ"}
+        for d, item in enumerate(geo.findall("DataItem")):
+            ref = item.text.strip()
+            assert ref == f"{fname}:/x{d}_ng"
+            assert item.attrib["Dimensions"] == dims[f"x{d}_ng"] == str(nx[d] + 1)
+            assert (item.attrib["NumberType"], item.attrib["Precision"], item.attrib["Format"]) == ("Float", "8", "HDF")
+        attrs = grid.findall("Attribute")
+        assert [a.attrib["Name"] for a in attrs] == want_vars
+        for a in attrs:
+            assert a.attrib["Center"] == "Cell" and a.attrib["AttributeType"] == "Scalar"
+            item = a.find("DataItem")
+            name = a.attrib["Name"]
+            assert item.text.strip() == f"{fname}:/{name}"
+            assert item.attrib["Dimensions"] == dims[name] == f"{nx[2]} {nx[1]} {nx[0]}"
+    # rewritten whole at each output: a shorter record leaves a shorter file
+    hio.write_xdmf(str(tmp_path), "run", g.Nx_glob_ng, nd, nfx, 3, record[:1])
+    assert len(ET.fromstring("\n".join(open(path).read().splitlines()[2:])).findall("Domain/Grid/Grid")) == 1
+
+
 # ---- the oracle's output schedule (main.cpp:362-385)
 def _sod_oracle(n=64):
     s = setups.sod(n)
