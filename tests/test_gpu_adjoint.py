@@ -324,6 +324,28 @@ def test_step_writes_every_cell_of_its_output(cuda):
             assert torch.equal(a, b)
 
 
+def test_auto_schedule_follows_the_memory_budget(cuda):
+    """schedule="auto": every state kept (no recomputation) when the N + 1 states fit in the budget,
+    periodic checkpoints (each segment recomputed once) when they do not; the same gradient."""
+    from heraclespp_amd.adjoint import _Rollout, state_bytes
+
+    s, spec, rho0, u, P = _smooth(32, 12)
+    gamma0 = torch.tensor(1.4, dtype=torch.float64)
+    r, m, E = _cons0(rho0, u, P, gamma0)
+    dts = cfl_schedule(spec, r, m, E, 1.4, 12, s.cfl)
+    target = torch.ones_like(rho0)
+    need = 13 * state_bytes(spec)
+    grads = []
+    for budget, recomputed in ((need, 0), (need - 1, 12 - 3)):
+        x = rho0.clone().requires_grad_(True)
+        r, m, E = _cons0(x, u, P, gamma0)
+        rT, _, _ = rollout(r, m, E, gamma0, spec, dts, checkpoint_every=5, memory_budget=budget)
+        (0.5 * torch.sum((rT - target) ** 2)).backward()
+        assert _Rollout.last_backward == {"reverse_steps": 12, "recomputed_steps": recomputed}
+        grads.append(x.grad.clone())
+    assert torch.equal(grads[0], grads[1])
+
+
 def test_batched_reverse_steps_equal_single_steps(cuda):
     """hclpp_steps_adjoint over 6 steps (after the first, each step's primitives come from the
     gather pass of the step before, and its dgamma partials are folded by the faces pass of the
