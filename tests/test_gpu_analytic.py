@@ -1,6 +1,8 @@
 """The GPU step against the analytic solutions of the reference's own test problems
 (test/exact_shock_tube.py, test/exact_sedov.py), independent of the oracle:
 
+* advection of a sine wave once around the periodic box (test/convergence_advection_sinus, 1D in
+  the reference's CI; here 1D-3D): the L1 error falls as h^2;
 * Sod (1000 cells, Minmod + HLL, CFL 0.4, t = 0.2): the fused step, FP64 (reference and fast
   arithmetic) and FP32 storage,
   within L1(rho) < 2e-3 of the exact Riemann solution (Toro ch. 4; star state pinned against
@@ -24,6 +26,50 @@ from oracle import oracle as orc
 from tests.helpers import make_solver
 
 pytestmark = pytest.mark.gpu
+
+
+def _advection(nd, n):
+    """rho = 1 + 0.1 sin(2 pi (x + y + z)) over the unit cube, u = 1 along every dim, P = 0.1,
+    periodic, VanLeer + HLLC, CFL 0.4, t_end 1: the profile is advected once around the box."""
+    s = setups.Setup(f"advection{nd}d", nd, [n if d < nd else 1 for d in range(3)], "Cartesian", (0.0, 0.0, 0.0),
+                     (1.0, 1.0, 1.0), "VanLeer", "HLLC", ["Periodic"] * (2 * nd), 1.4, 0.4, 1.0, 1 << 30)
+    s.rho = np.zeros(s.shape3)
+    s.u = np.zeros((nd,) + s.shape3)
+    s.P = np.full(s.shape3, 0.1)
+    xc = [(v[:-1] + v[1:]) / 2 for v in s.nodes()]
+    Z, Y, X = np.meshgrid(xc[2], xc[1], xc[0], indexing="ij")
+    phase = sum((X, Y, Z)[:nd])
+    s.rho[...] = 1.0 + 0.1 * np.sin(2 * np.pi * phase)
+    s.u[...] = 1.0
+    return s, phase
+
+
+@pytest.mark.parametrize("nd,sizes,precision", [(1, (50, 100, 200, 400), "fp64"), (1, (50, 100, 200, 400), "fp64_fast"),
+                                                (1, (50, 100, 200), "fp32"), (2, (32, 64, 128, 256), "fp64"),
+                                                (3, (32, 64, 128), "fp64"), (3, (32, 64, 128), "fp32")])
+def test_advection_is_second_order(cuda, nd, sizes, precision):
+    """The reference's convergence test (test/convergence_advection_sinus, run by its CI in 1D),
+    on the fused GPU step in 1D-3D: the L1 distance of rho to the exact cell averages after one
+    period, sin(2 pi (x + y + z)) (sin(pi h) / (pi h))^nd, falls as h^2 (slope within 5 %)."""
+    from heraclespp_amd import _lib
+
+    errs, hs = [], []
+    for n in sizes:
+        s, phase = _advection(nd, n)
+        sol = make_solver(s, precision=precision.replace("_fast", ""))
+        if precision == "fp64_fast":
+            _lib.check(_lib.lib().hclpp_set_fast_arithmetic(sol.ctx.handle, 1))
+        sol.init_from_prim(s.rho, s.u, s.P)
+        sol.run(s.t_end, 1 << 30)
+        assert abs(sol.t - 1.0) < 1e-14
+        h = 1.0 / n
+        exact = 1.0 + 0.1 * np.sin(2 * np.pi * phase) * (np.sin(np.pi * h) / (np.pi * h)) ** nd
+        it = s.interior()
+        errs.append(float(np.mean(np.abs(sol.rho.cpu().numpy()[it] - exact[it]))))
+        hs.append(h)
+    slope = np.polyfit(np.log(hs), np.log(errs), 1)[0]
+    print(f"[advection] {nd}D {precision} L1 {['%.3e' % e for e in errs]} slope {slope:.3f}", flush=True)
+    assert abs(slope - 2) / 2 < 0.05, (slope, errs)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "fp64_fast"])
