@@ -1,5 +1,5 @@
 """Write tests/golden/long_run_digests.json: the CPU oracle's state digests for the configured
-full-length runs, so the GPU box can check N-step bit-identity without minutes of host time.
+full-length runs, so a GPU test can check N-step bit-identity without minutes of host time.
 
 Runs (the oracle, OpenMP over this host's cores):
   * 2D Kelvin-Helmholtz 2048^2, VanLeer + HLLC, CFL 0.4, the configured 2000 steps;
