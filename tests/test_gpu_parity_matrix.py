@@ -1,4 +1,4 @@
-"""Parity of the fused step with the oracle over every scheme combination the factories accept:
+"""Parity of the hydro step with the oracle over every scheme combination the factories accept:
 1D-3D, Cartesian and spherical geometry, the four limiters (limited_linear_reconstruction.hpp /
 slope_limiters.hpp) and the three Riemann solvers (riemann_solver.hpp, "Low Mach" splitting
 included), each with its own seeded mix of boundary kinds and gravity (uniform g in Cartesian
@@ -6,7 +6,9 @@ runs, a point mass in half of the spherical ones).  Each case starts from a seed
 random state, so every limiter branch and wave pattern is exercised, and runs a few CFL steps.
 
 * FP64: every conservative array (ghosts included), t and the iteration count equal the
-  oracle's bit for bit, for every combination (each one a distinct instance of the march kernel).
+  oracle's bit for bit, for every combination (each one a distinct instance of the march kernel),
+  on the fused step, on the reference's operator sequence, and on the fused step carrying two
+  passive scalars (the one-scalar kernel instances).
 * FP32 storage: the density stays within a relative L1 distance of 1e-6 of the FP64 oracle.
 """
 
@@ -52,9 +54,9 @@ def _case(nd, geom, lim, rs):
     return s, kw
 
 
-def _oracle(s, kw):
-    o = make_oracle(s, **kw)
-    o.init_from_prim(s.rho, s.u, s.P)
+def _oracle(s, kw, fx=None):
+    o = make_oracle(s, nfx=0 if fx is None else len(fx), **kw)
+    o.init_from_prim(s.rho, s.u, s.P, *([] if fx is None else [fx]))
     for _ in range(STEPS):
         _, ex = o.step(s.cfl, s.t_end, STEPS)
         if ex:
@@ -62,18 +64,27 @@ def _oracle(s, kw):
     return o
 
 
+@pytest.mark.parametrize("path", ["fused", "operators", "fused_2_scalars"])
 @pytest.mark.parametrize("nd,geom,lim,rs", COMBOS, ids=[f"{c[0]}d-{c[1]}-{c[2]}-{c[3].replace(' ', '')}" for c in COMBOS])
-def test_fused_step_matches_oracle_bitwise(cuda, nd, geom, lim, rs):
+def test_step_matches_oracle_bitwise(cuda, nd, geom, lim, rs, path):
+    """fused: hclpp_step; operators: the reference kernel sequence (hclpp_hydro_reconstruction,
+    hclpp_godunov, ...); fused_2_scalars: hclpp_step_fx with two seeded passive scalars (mass
+    fractions in [0, 1]), one launch per scalar."""
     s, kw = _case(nd, geom, lim, rs)
-    print(f"\n[matrix] {s.name} n={s.n[:nd]} bc={s.bc} {kw or s.gravity}", flush=True)
-    o = _oracle(s, kw)
-    sol = make_solver(s, **kw)
-    assert sol.fused
-    sol.init_from_prim(s.rho, s.u, s.P)
+    fx = None
+    if path == "fused_2_scalars":
+        rng = np.random.default_rng(7)
+        fx = rng.random((2,) + s.shape3)
+    print(f"\n[matrix] {s.name} {path} n={s.n[:nd]} bc={s.bc} {kw or s.gravity}", flush=True)
+    o = _oracle(s, kw, fx)
+    sol = make_solver(s, fused=path != "operators", nfx=0 if fx is None else 2, **kw)
+    assert sol.fused == (path != "operators")
+    sol.init_from_prim(s.rho, s.u, s.P, *([] if fx is None else [fx]))
     steps = sol.run(s.t_end, STEPS)
     assert steps == o.iter == STEPS
     assert sol.t == o.t
-    for name, gpu in (("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E)):
+    arrays = [("rho", sol.rho), ("rhou", sol.rhou), ("E", sol.E)] + ([("fx", sol.fx)] if fx is not None else [])
+    for name, gpu in arrays:
         np.testing.assert_array_equal(gpu.cpu().numpy(), o.state(name), err_msg=name)
 
 
