@@ -74,8 +74,8 @@ struct GhostShell {
 template <int ND>
 struct MarchShape;
 template <>
-struct MarchShape<1> {  // 64 independent segments of 16 cells per workgroup
-    static constexpr int HX = 1, HY = 1, NT = 64, SEG = 16;
+struct MarchShape<1> {  // 64 independent segments per workgroup (plan_march sets their length)
+    static constexpr int HX = 1, HY = 1, NT = 64;
 };
 template <>
 struct MarchShape<2> {  // a 64-column row (62 outputs + halo) marching along y

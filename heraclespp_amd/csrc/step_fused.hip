@@ -262,13 +262,21 @@ MarchPlan plan_march(const GridDev& G, int nd, int blocks_per_cu, int ncu)
 {
     MarchPlan P{};
     int const nm = G.n[nd - 1];
-    if (nd == 1) {
+    if (nd == 1) {  // one segment per thread: the same cost model over NT-segment workgroups
         using S = MarchShape<1>;
-        P.L.seglen = S::SEG;
         P.L.ntile[0] = 1;
         P.L.ntile[1] = 1;
-        int const nseg = (nm + S::SEG - 1) / S::SEG;
-        P.nb = (nseg + S::NT - 1) / S::NT;
+        int64_t const resident = (int64_t)std::max(blocks_per_cu, 1) * std::max(ncu, 1);
+        int64_t best_cost = -1;
+        for (int seg = std::min(4, nm); seg <= std::min(std::max(nm, 1), 1 << 16); ++seg) {
+            int64_t const nb = ((nm + seg - 1) / seg + S::NT - 1) / S::NT;
+            int64_t const cost = ((nb + resident - 1) / resident) * (seg + 2);
+            if (best_cost < 0 || cost < best_cost) {
+                best_cost = cost;
+                P.L.seglen = seg;
+                P.nb = nb;
+            }
+        }
         return P;
     }
     int const HX = nd == 2 ? MarchShape<2>::HX : MarchShape<3>::HX;
